@@ -1,0 +1,401 @@
+"""Pins the FP64 CPU oracle to the reference's own known-answer tests and to
+checks that fix the mathematics independently of the oracle's formulas.
+
+Each test names the reference test it restates.  The reference ships no golden
+files; the committed fixtures under tests/golden/ come from numpy and from the
+Random123 publication (tests/golden/make_fixtures.py).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# --------------------------------------------------------------------- RNG
+def test_philox_published_kat():
+    with open(os.path.join(GOLD, "philox_kat.json")) as f:
+        kat = json.load(f)
+    for ctr, key, expected in kat:
+        assert O.philox(ctr, key) == expected
+
+
+def test_bootstrap_indices_range_and_uniformity():
+    # kde_tests.jl:17-20: size (n, B), integer, inside 1:n (0-based here)
+    idx = O.bootstrap_indices(seed=11, n=1000, B=40)
+    assert idx.shape == (40, 1000)
+    assert idx.min() >= 0 and idx.max() < 1000
+    counts = np.bincount(idx.ravel(), minlength=1000)
+    # each index expected 40 times; chi-square well inside its bulk
+    chi2 = ((counts - 40.0) ** 2 / 40.0).sum()
+    assert 800 < chi2 < 1200
+    assert not np.array_equal(idx[0], idx[1])
+
+
+# --------------------------------------------------------------------- FFT
+def test_fft_matches_numpy_golden():
+    d = np.load(os.path.join(GOLD, "fft_vectors.npz"))
+    keys = sorted(k[3:] for k in d.files if k.startswith("in_"))
+    assert len(keys) >= 10
+    for key in keys:
+        a = d["in_" + key]
+        f = O.fft_nd(a, -1)
+        b = O.fft_nd(a, +1)
+        scale = np.abs(d["fwd_" + key]).max()
+        assert np.abs(f - d["fwd_" + key]).max() <= 1e-13 * scale, key
+        assert np.abs(b - d["inv_" + key]).max() <= 1e-13 * np.abs(d["inv_" + key]).max(), key
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 12, 16, 30, 49])
+def test_fft_brute_force_dft(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    assert np.allclose(O.fft_nd(a, -1), W @ a, rtol=0, atol=1e-12 * np.abs(a).sum())
+
+
+def test_fft_round_trip_and_parseval():
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((6, 10, 9)) + 1j * rng.standard_normal((6, 10, 9))
+    f = O.fft_nd(a, -1)
+    back = O.fft_nd(f, +1) / a.size
+    assert np.abs(back - a).max() < 1e-13
+    assert math.isclose((np.abs(f) ** 2).sum(), a.size * (np.abs(a) ** 2).sum(), rel_tol=1e-12)
+
+
+# ----------------------------------------------------------------- binning
+def _reference_kat_sequences(n_dims):
+    """directspace_tests.jl:1-41 restated: samples 0.05 and 0.15 on every
+    axis, grid -5:0.1:5, indices [1 2; 2 1]."""
+    n = 101
+    spacing_sq = (0.1 ** n_dims) ** 2
+    dens = np.zeros((2,) + (n,) * n_dims)
+    dens2 = np.zeros_like(dens)
+    import itertools
+    for il in (50, 51):  # 0-based low index (Julia 51 / 52)
+        for corner in itertools.product((il, il + 1), repeat=n_dims):
+            term = (0.05 ** n_dims) / (2 * spacing_sq)
+            for b in (0, 1):
+                dens[(b,) + tuple(reversed(corner))] += term
+                dens2[(b,) + tuple(reversed(corner))] += term ** 2
+    return dens.reshape(2, -1), dens2.reshape(2, -1)
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_binning_reference_kat(n_dims):
+    g = O.make_grid_step([-5.0] * n_dims, [5.0] * n_dims, [0.1] * n_dims, [101] * n_dims)
+    samples = np.array([[0.05] * n_dims, [0.15] * n_dims])
+    idx = np.array([[0, 1], [1, 0]], dtype=np.int32)  # idx[b][s]
+    rc, sums, sqs = O.dirac_sequences(g, samples, idx)
+    assert rc == 0
+    exp, exp2 = _reference_kat_sequences(n_dims)
+    assert np.allclose(sums, exp, rtol=1e-12, atol=0)
+    assert np.allclose(sqs, exp2, rtol=1e-12, atol=0)
+
+
+def _tent_deposit(g, samples, idx_col):
+    """Independent formulation of linear binning: each node j receives
+    prod_i (h_i - |x_i - node_i|)_+ / (n (prod h)^2)."""
+    shape = O.grid_shape(g)
+    N = g.ndims
+    hs = np.array([g.h[i] for i in range(N)])
+    los = np.array([g.lo[i] for i in range(N)])
+    nodes = [los[i] + hs[i] * np.arange(shape[i]) for i in range(N)]
+    n = len(idx_col)
+    out = np.zeros(tuple(reversed(shape)))
+    out2 = np.zeros_like(out)
+    for s in idx_col:
+        x = samples[s]
+        ws = [np.clip(hs[i] - np.abs(x[i] - nodes[i]), 0, None) for i in range(N)]
+        w = ws[0]
+        for i in range(1, N):
+            w = np.multiply.outer(ws[i], w)
+        term = w / (n * np.prod(hs) ** 2)
+        out += term
+        out2 += term ** 2
+    return out.ravel(), out2.ravel()
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_binning_asymmetric_brute_force(n_dims):
+    rng = np.random.default_rng(40 + n_dims)
+    shape = [13, 9, 7][:n_dims]
+    lo = [-1.5, -0.7, -2.0][:n_dims]
+    hi = [2.5, 1.3, 1.0][:n_dims]
+    g = O.make_grid(lo, hi, shape)
+    n = 25
+    samples = np.column_stack([rng.uniform(lo[i] + 1e-3, hi[i] - (hi[i] - lo[i]) / (shape[i] - 1) - 1e-3, n)
+                               for i in range(n_dims)])
+    idx = rng.integers(0, n, size=(3, n)).astype(np.int32)
+    rc, sums, sqs = O.dirac_sequences(g, samples, idx)
+    assert rc == 0
+    for b in range(3):
+        e1, e2 = _tent_deposit(g, samples, idx[b])
+        assert np.allclose(sums[b], e1, rtol=1e-12, atol=1e-14 * e1.max())
+        assert np.allclose(sqs[b], e2, rtol=1e-11, atol=1e-14 * e2.max())
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_binning_mass_and_first_moment(n_dims):
+    """Linear binning keeps total mass 1/prod(h) and the first moment along
+    every axis (catches swapped axes in the weights)."""
+    rng = np.random.default_rng(5 + n_dims)
+    shape = [40, 30, 20][:n_dims]
+    lo = [-3.0, -2.0, -1.0][:n_dims]
+    hi = [3.0, 4.0, 1.5][:n_dims]
+    g = O.make_grid(lo, hi, shape)
+    n = 500
+    samples = np.column_stack([rng.uniform(lo[i] + 0.01, hi[i] - 0.2, n) for i in range(n_dims)])
+    idx = rng.integers(0, n, size=(2, n)).astype(np.int32)
+    rc, sums, _ = O.dirac_sequences(g, samples, idx, with_squares=False)
+    hp = np.prod([g.h[i] for i in range(n_dims)])
+    for b in range(2):
+        f = sums[b].reshape(tuple(reversed(shape)))
+        assert math.isclose(f.sum(), 1.0 / hp, rel_tol=1e-12)
+        for ax in range(n_dims):
+            coords = lo[ax] + g.h[ax] * np.arange(shape[ax])
+            axis_np = n_dims - 1 - ax
+            bshape = [1] * n_dims
+            bshape[axis_np] = shape[ax]
+            m1 = (f * coords.reshape(bshape)).sum()
+            assert math.isclose(m1, samples[idx[b], ax].mean() / hp, rel_tol=1e-10, abs_tol=1e-10 / hp)
+
+
+def test_binning_out_of_grid_rejected():
+    g = O.make_grid([-1.0], [1.0], [21])
+    for bad in (1.0, -1.0, 1.5, np.nextafter(1.0, 0.0)):
+        rc, _, _ = O.dirac_sequences(g, np.array([[0.0], [bad]]), np.array([[0, 1]], dtype=np.int32))
+        assert rc == 2, bad
+
+
+# --------------------------------------------------------------------- VMR
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_vmr_closed_form(n_dims):
+    """directspace_tests.jl:47-61,210-230."""
+    d = np.load(os.path.join(GOLD, "vmr_closed_form.npz"))
+    m, v, exp = d[f"mean_{n_dims}"], d[f"var_{n_dims}"], d[f"expected_{n_dims}"]
+    B = m.shape[0]
+    G = int(np.prod(m.shape[1:]))
+    out = np.zeros(G)
+    sk = np.ascontiguousarray(m.reshape(B, G)).view(np.float64).ravel().copy()
+    s2k = np.ascontiguousarray(v.reshape(B, G)).view(np.float64).ravel().copy()
+    O.lib().og_scaled_vmr(n_dims, G, B, sk, s2k, np.full(n_dims, 1.0), np.full(n_dims, 0.0), 2, out)
+    e = exp.reshape(G)
+    fin = np.isfinite(e)
+    assert np.array_equal(fin, np.isfinite(out))
+    assert np.allclose(out[fin], e[fin], rtol=1e-10, atol=1e-12)
+
+
+def test_full_means_abs():
+    """directspace_tests.jl:63-67,232-245."""
+    rng = np.random.default_rng(9)
+    z = rng.random(500) + 1j * rng.random(500)
+    out = np.zeros(500)
+    O.lib().og_full_means(500, z.view(np.float64).copy(), out)
+    assert np.allclose(out, np.abs(z), rtol=1e-15)
+
+
+# ------------------------------------------------------------- convergence
+def _snapshot(stage, G, steps_low):
+    """directspace_tests.jl:69-128 (CPU snapshots)."""
+    if stage == "low_density_recon":
+        return (np.full(G, 1.0e6), np.full(G, 0.8), np.full(G, 1.0e6), np.zeros(G),
+                np.full(G, steps_low, np.uint16), np.zeros(G, np.uint16), np.zeros(G, np.uint8))
+    if stage == "low_density_stop":
+        return (np.full(G, 1.0), np.full(G, 0.99), np.full(G, 1.0), np.full(G, np.nan),
+                np.full(G, steps_low + 1, np.uint16), np.zeros(G, np.uint16), np.ones(G, np.uint8))
+    if stage == "high_density_update":
+        return (np.full(G, 1.00005), np.full(G, 1.0), np.full(G, 1.00005), np.zeros(G),
+                np.zeros(G, np.uint16), np.zeros(G, np.uint16), np.zeros(G, np.uint8))
+    if stage == "high_density_noupdate":
+        return (np.full(G, 1.01), np.full(G, 1.0), np.full(G, 1.01), np.full(G, -5.0),
+                np.zeros(G, np.uint16), np.zeros(G, np.uint16), np.zeros(G, np.uint8))
+    raise ValueError(stage)
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_convergence_snapshots(n_dims):
+    """directspace_tests.jl:247-368: dlogt 0.1, eps 2, steps_low 3, steps_over 30."""
+    G = 101 ** n_dims if n_dims < 3 else 21 ** 3
+    dlogt, eps, sl, so = 0.1, 2.0, 3, 30
+    L = O.lib()
+
+    def run(stage):
+        cur, p1, p2, mins, cl, co, lf = _snapshot(stage, G, sl)
+        dens = np.full(G, np.nan)
+        means = np.ones(G)
+        L.og_identify_convergence(G, dens, means, cur, p1, p2, dlogt, eps, sl, so, mins, cl, co, lf,
+                                  None, 0, 0)
+        return dens, mins, cl, co, lf
+
+    dens, mins, cl, co, lf = run("low_density_recon")
+    assert np.all(np.isnan(dens)) and np.all(np.isnan(mins))
+    assert np.all(cl == sl + 1) and np.all(co == 0) and np.all(lf == 1)
+
+    dens, mins, cl, co, lf = run("low_density_stop")
+    assert np.all(dens == 1.0) and not np.any(mins == 0)
+    assert np.all(cl == sl + 1) and np.all(co == 0) and np.all(lf == 1)
+
+    dens, mins, cl, co, lf = run("high_density_update")
+    assert np.all(dens == 1.0) and np.all(mins <= 0)
+    assert np.all(cl == 0) and np.all(co == 0) and np.all(lf == 0)
+
+    dens, mins, cl, co, lf = run("high_density_noupdate")
+    assert np.all(np.isnan(dens)) and np.all(mins == -5.0)
+    assert np.all(cl == 0) and np.all(co == 1) and np.all(lf == 0)
+
+
+def test_history_rotation():
+    """gradepro_tests.jl:127-164: f_prev2 <- f_prev1 <- vmr after a step."""
+    G = 50
+    vmr = np.full(G, 0.25)
+    p1, p2 = np.ones(G), np.full(G, 2.0)
+    mins, cl, co, lf = np.full(G, np.nan), np.zeros(G, np.uint16), np.zeros(G, np.uint16), np.zeros(G, np.uint8)
+    O.lib().og_identify_convergence(G, np.full(G, np.nan), np.ones(G), vmr, p1, p2, 1e-4, 2.0, 3, 30,
+                                    mins, cl, co, lf, None, 0, 1)
+    assert np.all(p1 == 0.25) and np.all(p2 == 1.0)
+
+
+# ------------------------------------------------------------ ladder/grid
+def test_ladder_time_step_kat():
+    """gradepro_tests.jl:166-177: time_final 2.0, step 0.2."""
+    for n_dims in (1, 2, 3):
+        bw = np.full(n_dims, 2.0)
+        S = O.lib().og_ladder(n_dims, bw, 0, 0.2, None, None)
+        times = np.zeros(S * n_dims)
+        dt = np.zeros(n_dims)
+        O.lib().og_ladder(n_dims, bw, 0, 0.2, dt.ctypes.data, times.ctypes.data)
+        times = times.reshape(S, n_dims)
+        assert np.all(times[0] == 0.0) and np.all(times[-1] == 2.0) and np.all(dt == 0.2)
+    assert O.lib().og_ladder(1, np.array([1.0]), 0, 1.5, None, None) == -1
+
+
+def test_ladder_defaults_and_duration_steps():
+    """GradeProEstimator.jl:754-766,954-966: (steps_low, steps_over) is
+    (0, 25) in 1-D and (15, 25) in 2-D/3-D with 250 steps."""
+    for n_dims, alpha_s, expect in ((1, 0.0, (0, 25)), (2, 0.06, (15, 25)), (3, 0.06, (15, 25))):
+        bw = np.linspace(0.11, 0.37, n_dims)
+        S = O.lib().og_ladder(n_dims, bw, 250, np.nan, None, None)
+        assert S == 251
+        times = np.zeros(S * n_dims)
+        dt = np.zeros(n_dims)
+        O.lib().og_ladder(n_dims, bw, 250, np.nan, dt.ctypes.data, times.ctypes.data)
+        import ctypes as C
+        sl, so = C.c_int(), C.c_int()
+        O.lib().og_duration_steps(n_dims, times[-n_dims:].copy(), dt, alpha_s, 0.1, C.byref(sl), C.byref(so))
+        assert (sl.value, so.value) == expect
+
+
+def test_dlogt_first_rung_is_zero():
+    t0 = np.array([0.05, 0.02])
+    assert O.lib().og_dlogt(2, t0, np.zeros(2), None) == 0.0
+    t1, t2 = np.array([0.1, 0.2]), np.array([0.2, 0.4])
+    expect = math.log((0.0025 + 0.04) * (0.0004 + 0.16) / ((0.0025 + 0.01) * (0.0004 + 0.04)))
+    assert math.isclose(O.lib().og_dlogt(2, t0, t2, t1.ctypes.data), expect, rel_tol=1e-14)
+
+
+def test_fftgrid_matches_numpy():
+    """grid_tests.jl:22-27: fftgrid = 2pi fftfreq(n, 1/h)."""
+    for n, h in ((21, 0.1), (512, 10 / 511), (300, 12 / 299), (7, 0.5)):
+        assert np.allclose(O.fftfreq(n, h), 2 * np.pi * np.fft.fftfreq(n, h), rtol=1e-15, atol=1e-15)
+
+
+def test_silverman_closed_form():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(1000, 2)) * np.array([1.0, 3.0])
+    bw = O.silverman(x)
+    pref = (4.0 / (4 * 1000)) ** (1 / 6)
+    assert np.allclose(bw, pref * x.std(axis=0, ddof=1), rtol=1e-13)
+
+
+# ------------------------------------------------------------- propagation
+def _direct_kde(samples, nodes, sigma):
+    """Direct Gaussian KDE on the grid and its squared-kernel companion
+    (fourierspace_tests.jl:1-25)."""
+    n, N = samples.shape
+    dens = 0.0
+    dens2 = 0.0
+    for s in range(n):
+        k = 1.0
+        for i in range(N):
+            k = np.multiply.outer(np.exp(-0.5 * ((nodes[i] - samples[s, i]) / sigma) ** 2) /
+                                  (np.sqrt(2 * np.pi) * sigma), k) if i > 0 else \
+                np.exp(-0.5 * ((nodes[i] - samples[s, i]) / sigma) ** 2) / (np.sqrt(2 * np.pi) * sigma)
+        k = k / n
+        dens = dens + k
+        dens2 = dens2 + k ** 2
+    return np.asarray(dens), np.asarray(dens2)
+
+
+@pytest.mark.parametrize("n_dims", [1, 2])
+def test_propagation_gaussian_semigroup(n_dims):
+    """fourierspace_tests.jl:27-96: a KDE at sigma=0.2 propagated by t=0.2
+    equals the KDE at sqrt(2)*0.2; the squared field within rtol 1e-6."""
+    rng = np.random.default_rng(100 + n_dims)
+    samples = rng.standard_normal((100, n_dims))
+    n = 101
+    g = O.make_grid_step([-5.0] * n_dims, [5.0] * n_dims, [0.1] * n_dims, [n] * n_dims)
+    nodes = [np.linspace(-5.0, 5.0, n)] * n_dims
+    d0, d20 = _direct_kde(samples, nodes, 0.2)
+    d1, d21 = _direct_kde(samples, nodes, np.sqrt(2 * 0.2 ** 2))
+    f0 = O.fft_nd(d0.astype(complex), -1)
+    f20 = O.fft_nd(d20.astype(complex), -1)
+    kg = np.concatenate([O.fftfreq(n, 0.1)] * n_dims)
+    G = n ** n_dims
+    mt = np.zeros(2 * G)
+    vt = np.zeros(2 * G)
+    t = np.full(n_dims, 0.2)
+    import ctypes
+    m0 = f0.ravel().view(np.float64).copy()
+    v0 = f20.ravel().view(np.float64).copy()
+    O.lib().og_propagate(ctypes.byref(g), kg, t, t, m0, v0.ctypes.data, mt, vt.ctypes.data)
+    m = mt.view(np.complex128).reshape(d0.shape)
+    v = vt.view(np.complex128).reshape(d0.shape)
+    f1 = O.fft_nd(d1.astype(complex), -1)
+    f21 = O.fft_nd(d21.astype(complex), -1)
+    assert np.allclose(m, f1, rtol=1e-6, atol=1e-8 * np.abs(f1).max())
+    assert np.allclose(v, f21, rtol=1e-6, atol=1e-6 * np.abs(f21).max())
+    back = np.abs(O.fft_nd(m, +1) / G)
+    assert np.allclose(back, d1, rtol=1e-6, atol=1e-8 * d1.max())
+
+
+# -------------------------------------------------------------- end to end
+def test_end_to_end_mise_1d():
+    """gradepro_tests.jl:485-511 (N=1): 10,000 N(0,1) samples, grid
+    range(-6, 6, length=300), MISE < 5e-5 (seeded here)."""
+    from scipy.stats import norm
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((10000, 1))
+    g = O.make_grid([-6.0], [6.0], [300])
+    r = O.gradepro(g, x, seed=5, threads=4)
+    assert r.rc == 0 and r.info.n_rungs == 251
+    xs = np.linspace(-6, 6, 300)
+    mise = ((r.density - norm.pdf(xs)) ** 2).sum() * (xs[1] - xs[0]) / 300
+    assert mise < 5e-5
+    assert np.all(np.isfinite(r.density))
+
+
+def test_serial_equals_threaded():
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((3000, 2)) * 0.8
+    g = O.make_grid([-5.0, -5.0], [5.0, 5.0], [24, 20])
+    a = O.gradepro(g, x, seed=3, n_bootstraps=16, n_steps=40, threads=1)
+    b = O.gradepro(g, x, seed=3, n_bootstraps=16, n_steps=40, threads=4)
+    assert np.array_equal(a.density, b.density) and np.array_equal(a.chosen, b.chosen)
+
+
+def test_chosen_rung_consistent_with_means_table():
+    """The chosen rung reproduces the density from the per-rung means."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((2000, 1))
+    g = O.make_grid([-5.0], [5.0], [128])
+    r = O.gradepro(g, x, seed=1, n_bootstraps=20, n_steps=60, tables=True)
+    S = r.info.n_rungs
+    sel = np.where(r.chosen == S, S - 1, r.chosen)
+    assert np.array_equal(r.density, r.means[sel, np.arange(128)])
