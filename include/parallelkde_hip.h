@@ -1,0 +1,196 @@
+/*
+ * libparallelkde_hip -- C ABI of the MI355X (gfx950) :gradepro density
+ * estimator.  Plain C types only: no torch, HIP or C++ types cross it.
+ *
+ * It is the drop-in for ParallelKDE.jl's `estimate_density!(est, :gradepro)`
+ * on a grid.  The Julia glue behind a new `:rocm` device calls it through
+ * ccall; Python reaches it through ctypes.  Citations are relative to the
+ * ParallelKDE.jl v1.0.1 source tree.
+ *
+ * Layouts follow Julia's column-major convention, so the glue never
+ * transposes anything:
+ *   samples       ndims x n, coordinate fastest (KDEs.jl:76-93 keeps the same
+ *                 order as Vector{SVector{N}}).
+ *   bootstrap_idx n x B, draw fastest, 0-based (KDEs.jl:182-191 returns the
+ *                 1-based n x B matrix; the glue subtracts 1).
+ *   density,
+ *   chosen_rung   grid axis 1 fastest (Julia Array{T,N} order).
+ *
+ * Status codes map onto the reference's error behaviour:
+ *   PKDE_INVALID_ARGUMENT, PKDE_OUT_OF_GRID   -> ArgumentError
+ *       (DirectSpace.jl:489-492, GradeProEstimator.jl:712-714, 891-893, 955-960)
+ *   PKDE_NO_DEVICE                           -> @warn + fall back to :cpu
+ *       (ParallelKDE.jl:123-127)
+ *   PKDE_OUT_OF_MEMORY                       -> ArgumentError("Not enough memory...")
+ *       (GradeProEstimator.jl:1105-1108)
+ * pkde_last_error() returns a thread-local message for the last failure.
+ */
+#ifndef PARALLELKDE_HIP_H
+#define PARALLELKDE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PKDE_ABI_VERSION 1
+#define PKDE_MAX_DIMS 3
+
+typedef enum {
+  PKDE_OK = 0,
+  PKDE_INVALID_ARGUMENT = 1,
+  PKDE_OUT_OF_GRID = 2,
+  PKDE_NO_DEVICE = 3,
+  PKDE_OUT_OF_MEMORY = 4,
+  PKDE_DEVICE_ERROR = 5,
+  PKDE_UNSUPPORTED = 6,
+  PKDE_PENDING = 7
+} pkde_status;
+
+typedef struct pkde_ctx pkde_ctx;
+typedef struct pkde_plan pkde_plan;
+typedef struct pkde_server pkde_server;
+
+/* Keyword arguments of estimate_density!(est, :gradepro; ...)
+ * (docs/src/estimators/gradepro.md:19-28, GradeProEstimator.jl:700-796).
+ * NaN / <= 0 select the reference default. */
+typedef struct {
+  int32_t n_bootstraps;                /* 100 (:705) */
+  int32_t n_steps;                     /* 250 (:902); <= 0 -> bw_step if finite */
+  double bw_step;                      /* NaN: unset (:887-899) */
+  double bw_final[PKDE_MAX_DIMS];      /* NaN: Silverman rule (:771-775) */
+  double alpha_s;                      /* NaN: 0.0 (1-D) / 0.06 (>=2-D) (:754-763) */
+  double alpha_os;                     /* NaN: 0.1 (:764-766) */
+  double eps;                          /* NaN: 2.0 (1-D) / -0.5 (>=2-D) (:541-550) */
+  uint64_t rng_seed;                   /* Philox key for in-kernel bootstrap draws */
+  int32_t deterministic_binning;       /* 1: fixed-point integer deposits (default) */
+  int32_t precision;                   /* 0: FP32 spectra + FFT, FP64 statistics
+                                          1: FP64 throughout */
+} pkde_gradepro_params;
+
+/* Timing / accounting of one estimate (optional output). */
+typedef struct {
+  int32_t n_rungs;
+  int32_t steps_low, steps_over;
+  double eps;
+  double bw_final[PKDE_MAX_DIMS];
+  double ms_total;          /* device time of the whole estimate */
+  double ms_ladder;         /* device time of the ladder kernels */
+  double algorithmic_bytes; /* U*(8B+4) + n*N*8 + (2B+1)*G*4 + G*6 */
+} pkde_stats;
+
+int pkde_version(void);
+const char *pkde_last_error(void);
+void pkde_gradepro_params_default(pkde_gradepro_params *p);
+
+/* One context per device: owns the stream and the autotune cache.
+ * Replaces the reference's device registry entry (Devices.jl:54-82) for the
+ * new :rocm device.  Returns PKDE_NO_DEVICE without a usable GPU. */
+int pkde_context_create(int device, pkde_ctx **out);
+void pkde_context_destroy(pkde_ctx *ctx);
+/* Free device memory in bytes (Devices.jl:103-108, get_available_memory). */
+int pkde_device_free_memory(pkde_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* A plan binds a grid (Grid: Grids.jl:67-87 -- per-axis lo, hi, spacing and
+ * point count) and the estimator keywords.  Axis sizes must be powers of two
+ * in [2, 4096]; ndims in 1..3 (the reference's GPU path is capped at 3-D,
+ * GradeProEstimator.jl:811-814).  Replaces initialize_estimator
+ * (GradeProEstimator.jl:700-735) for the :rocm device. */
+int pkde_gradepro_plan_create(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
+                              const double *lo, const double *hi, const double *h,
+                              const pkde_gradepro_params *params, pkde_plan **out);
+void pkde_gradepro_plan_destroy(pkde_plan *plan);
+/* Device bytes an estimate of n samples needs (extends get_necessary_memory,
+ * GradeProEstimator.jl:1091-1096). */
+uint64_t pkde_gradepro_plan_footprint(const pkde_plan *plan, int64_t n_samples);
+
+/* The plain form: estimate!(::GradeProEstimator, kde) (GradeProEstimator.jl:
+ * 968-1051) + replace_nans! (:1053-1072).  Synchronous; copies the host
+ * samples in, writes caller-owned host outputs.  bootstrap_idx may be NULL
+ * (in-kernel Philox draws keyed by rng_seed).  chosen_rung_out (nullable):
+ * rung 0..S-1 of the last density assignment, or S where the final-rung
+ * fallback was used.  stats (nullable). */
+int pkde_gradepro_estimate(pkde_plan *plan, const double *samples, int64_t n,
+                           const int32_t *bootstrap_idx, double *density_out,
+                           int16_t *chosen_rung_out, pkde_stats *stats);
+
+/* Same on device-resident buffers, asynchronous on `stream` (a hipStream_t,
+ * NULL: the context stream).  Nothing is copied to or from the host.
+ * Out-of-grid samples are reported through pkde_gradepro_check(). */
+int pkde_gradepro_estimate_device(pkde_plan *plan, const double *d_samples, int64_t n,
+                                  const int32_t *d_bootstrap_idx, double *d_density,
+                                  int16_t *d_chosen_rung, void *stream);
+/* Waits for the plan's last device estimate; PKDE_OUT_OF_GRID if it saw a
+ * sample outside the grid (check_data, DirectSpace.jl:479-498). */
+int pkde_gradepro_check(pkde_plan *plan);
+/* Timing and ladder scalars of the plan's last estimate (waits for it). */
+int pkde_gradepro_last_stats(pkde_plan *plan, pkde_stats *stats);
+
+/* Per-rung log-VMR table of the last estimate (S x G, rung-major), for
+ * primitive-level parity (calculate_scaled_vmr!, DirectSpace.jl:526-584).
+ * Only available when the plan was created with debug tables enabled. */
+int pkde_gradepro_debug_enable(pkde_plan *plan, int enable);
+int pkde_gradepro_debug_vmr(pkde_plan *plan, double *vmr_out, int64_t capacity);
+
+/* Convergence state machine alone (identify_convergence! + find_stability,
+ * DirectSpace.jl:829-871, 975-1031) over a given S x G log-VMR table with the
+ * given per-rung dlogt; writes chosen rung per point (S = fallback). */
+int pkde_gradepro_convergence(pkde_ctx *ctx, int64_t G, int32_t S, const double *vmr,
+                              const double *dlogt, double eps, int32_t steps_low,
+                              int32_t steps_over, int16_t *chosen_out);
+
+/* Linear binning alone (initialize_dirac_sequence + generate_dirac_cpu!,
+ * DirectSpace.jl:97-145, 196-242) for primitive-level parity: writes the
+ * B x G sums and squares fields (real parts) in FP64. */
+int pkde_dirac_sequences(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
+                         const double *lo, const double *hi, const double *h,
+                         const double *samples, int64_t n, const int32_t *bootstrap_idx,
+                         int32_t B, int deterministic, double *sums_out, double *squares_out);
+
+/* ---- batched form: many independent estimates in one submission ----
+ * Every request's kernels go to one stream in sequence without host
+ * synchronisation; results are written when the call returns. */
+typedef struct {
+  int32_t ndims;
+  int64_t n_per_axis[PKDE_MAX_DIMS];
+  double lo[PKDE_MAX_DIMS], hi[PKDE_MAX_DIMS], h[PKDE_MAX_DIMS];
+  const double *samples; /* host, ndims x n */
+  int64_t n;
+  pkde_gradepro_params params;
+  double *density_out;   /* host, G */
+  int16_t *chosen_out;   /* host, G, nullable */
+} pkde_request;
+
+int pkde_gradepro_estimate_batch(pkde_ctx *ctx, int32_t count, const pkde_request *reqs,
+                                 int32_t *status_out);
+
+/* ---- serving form: continuous batching with HBM-footprint admission ----
+ * A server thread admits requests while their footprint fits the budget and
+ * a stream slot is free; requests that arrive while others run overlap on the
+ * device.  Results do not depend on what else is in flight. */
+typedef struct {
+  int32_t max_inflight;      /* concurrent requests (device streams) */
+  uint64_t memory_budget;    /* bytes of device memory for admitted requests */
+} pkde_server_cfg;
+
+typedef struct {
+  uint64_t completed, rejected, queued, inflight;
+  uint64_t bytes_in_use;
+  double p50_ms, p99_ms;     /* submit -> completion latency */
+} pkde_server_stats;
+
+int pkde_server_create(pkde_ctx *ctx, const pkde_server_cfg *cfg, pkde_server **out);
+/* Copies the request's samples; the caller keeps ownership of its outputs,
+ * which must stay valid until the ticket completes. */
+int pkde_server_submit(pkde_server *srv, const pkde_request *req, uint64_t *ticket);
+/* done = 1 when finished; returns the request's status once done. */
+int pkde_server_poll(pkde_server *srv, uint64_t ticket, int *done);
+int pkde_server_wait(pkde_server *srv, uint64_t ticket);
+int pkde_server_stats_get(pkde_server *srv, pkde_server_stats *out);
+void pkde_server_destroy(pkde_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

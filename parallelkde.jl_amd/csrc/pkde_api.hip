@@ -1,0 +1,649 @@
+// C ABI of libparallelkde_hip: contexts, plans and the plain estimate.
+// The batched and serving forms live in pkde_serve.hip.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "pkde_plan.hpp"
+
+using namespace pkde;
+
+namespace pkde {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PKDE_OK;
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return fail(PKDE_OUT_OF_MEMORY, m);
+  return fail(PKDE_DEVICE_ERROR, m);
+}
+
+#define PKDE_TRY(expr, what)                     \
+  do {                                           \
+    int _rc = check_hip((expr), what);           \
+    if (_rc != PKDE_OK) return _rc;              \
+  } while (0)
+
+static int dalloc(void** p, size_t bytes, const char* what) {
+  if (*p) return PKDE_OK;
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  return check_hip(hipMalloc(p, bytes), what);
+}
+
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static bool is_pow2(int64_t v) { return v >= 2 && (v & (v - 1)) == 0; }
+
+static int ilog2(int64_t v) {
+  int l = 0;
+  while ((1ll << l) < v) ++l;
+  return l;
+}
+
+static size_t real_size(int prec) { return prec ? sizeof(double) : sizeof(float); }
+
+// fixed-point fraction bits: deposits of n draws must fit in int64
+static int frac_bits_for(int64_t n) {
+  int lg = 0;
+  while ((1ll << lg) <= n) ++lg;
+  int fb = 61 - lg;
+  return fb > 40 ? 40 : fb;
+}
+
+static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; }
+
+uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
+  const int64_t G = pl->g.G, F = 2 * (int64_t)pl->B + 1, fs = fsize(pl);
+  const size_t rs = real_size(pl->prec);
+  uint64_t b = 0;
+  b += (uint64_t)n * pl->g.N * 8;                 // samples
+  b += (uint64_t)F * G * 8;                       // binning accumulators
+  b += (uint64_t)F * fs * 16;                     // FP64 spectra
+  if (!pl->prec) b += (uint64_t)F * fs * 8;       // FP32 spectra
+  b += (uint64_t)pl->S * G * 8;                   // log-VMR table
+  b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
+  b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
+  if (pl->g.N >= 2) {
+    const uint64_t per_rung = (uint64_t)2 * pl->B * fs * 2 * rs;
+    uint64_t Y = std::min<uint64_t>((uint64_t)pl->S * per_rung, std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes));
+    Y = std::max<uint64_t>(Y, (uint64_t)fs * 2 * rs);
+    b += Y;
+  }
+  return b;
+}
+
+int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
+  PlanBuffers& d = pl->d;
+  const int64_t G = pl->g.G, F = 2 * (int64_t)pl->B + 1, fs = fsize(pl);
+  const size_t rs = real_size(pl->prec);
+  if (n > d.x_cap) {
+    dfree(d.x);
+    d.x = nullptr;
+    int rc = dalloc((void**)&d.x, (size_t)n * pl->g.N * sizeof(double), "samples");
+    if (rc) return rc;
+    d.x_cap = n;
+  }
+  if (need_idx && n * (int64_t)pl->B > d.idx_cap) {
+    dfree(d.idx);
+    d.idx = nullptr;
+    int rc = dalloc((void**)&d.idx, (size_t)n * pl->B * sizeof(int32_t), "bootstrap indices");
+    if (rc) return rc;
+    d.idx_cap = n * (int64_t)pl->B;
+  }
+  int rc = PKDE_OK;
+  if ((rc = dalloc(&d.acc, (size_t)F * G * 8, "binning accumulators"))) return rc;
+  if ((rc = dalloc((void**)&d.specD, (size_t)F * fs * sizeof(cplx<double>), "spectra"))) return rc;
+  if (!pl->prec && (rc = dalloc(&d.spec, (size_t)F * fs * sizeof(cplx<float>), "spectra fp32"))) return rc;
+  if (pl->g.N >= 2 && !d.Y) {
+    const uint64_t per_rung = (uint64_t)2 * pl->B * fs * 2 * rs;
+    uint64_t Y = std::min<uint64_t>((uint64_t)pl->S * per_rung, std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes));
+    Y = std::max<uint64_t>(Y, (uint64_t)fs * 2 * rs);
+    if ((rc = dalloc(&d.Y, Y, "rung chunk buffer"))) return rc;
+    d.Y_elems = (int64_t)(Y / (2 * rs));
+  }
+  if (d.S_alloc < pl->S) {
+    dfree(d.vmr); dfree(d.times); dfree(d.dlogt); dfree(d.vscale); dfree(d.cV); dfree(d.gM); dfree(d.gV);
+    d.vmr = nullptr; d.times = nullptr; d.dlogt = nullptr; d.vscale = nullptr; d.cV = nullptr;
+    d.gM = nullptr; d.gV = nullptr;
+    const int S = pl->S;
+    if ((rc = dalloc((void**)&d.vmr, (size_t)S * G * 8, "vmr table"))) return rc;
+    if ((rc = dalloc((void**)&d.times, (size_t)S * PKDE_MAX_DIMS * 8, "times"))) return rc;
+    if ((rc = dalloc((void**)&d.dlogt, (size_t)S * 8, "dlogt"))) return rc;
+    if ((rc = dalloc((void**)&d.vscale, (size_t)S * 8, "vscale"))) return rc;
+    if ((rc = dalloc((void**)&d.cV, (size_t)S * 8, "cV"))) return rc;
+    if ((rc = dalloc(&d.gM, (size_t)S * pl->g.ksum * rs, "gauss M"))) return rc;
+    if ((rc = dalloc(&d.gV, (size_t)S * pl->g.ksum * rs, "gauss V"))) return rc;
+    d.S_alloc = S;
+  }
+  if ((rc = dalloc((void**)&d.chosen, (size_t)G * 2, "chosen"))) return rc;
+  if ((rc = dalloc((void**)&d.sel, (size_t)G * 2, "sel"))) return rc;
+  if ((rc = dalloc((void**)&d.density, (size_t)G * 8, "density"))) return rc;
+  if ((rc = dalloc((void**)&d.err, 256, "err"))) return rc;
+  if ((rc = dalloc((void**)&d.params, 256, "params"))) return rc;
+  if ((rc = dalloc((void**)&d.mean, 256, "mean"))) return rc;
+  if ((rc = dalloc((void**)&d.partial, (size_t)kMomentBlocks * PKDE_MAX_DIMS * 8, "partial"))) return rc;
+  return PKDE_OK;
+}
+
+static void plan_free(pkde_plan* pl) {
+  PlanBuffers& d = pl->d;
+  void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.chosen, d.sel, d.density, d.err,
+                d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
+                d.gM, d.gV};
+  for (void* p : ps) dfree(p);
+  d = PlanBuffers();
+  for (auto& e : pl->ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+template <typename T>
+static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
+                         double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
+  PlanBuffers& d = pl->d;
+  const GridDev& g = pl->g;
+  const int B = pl->B, S = pl->S, N = g.N;
+  const int64_t G = g.G, F = 2 * (int64_t)B + 1, fs = fsize(pl);
+  const T* tw = (const T*)(sizeof(T) == 8 ? (const void*)pl->ctx->tw64 : (const void*)pl->ctx->tw32);
+
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
+  launch_fill_i32(d.err, 0, 1, s);
+  launch_zero_bytes(d.acc, F * G * 8, s);
+  if (!pl->bw_given) {
+    const double pref = pow(4.0 / ((double)(N + 2) * (double)n), 1.0 / (double)(N + 4));
+    launch_silverman(N, d_x, n, pref, d.partial, d.mean, d.bw, s);
+  }
+  launch_tables<T>(g, S, pl->steps_mode, pl->steps_or_step, d.bw, d.kfreq, n, pl->alpha_s,
+                   pl->alpha_os, pl->eps, d.times, d.dlogt, d.vscale, d.cV, d.params, (T*)d.gM,
+                   (T*)d.gV, s);
+  launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err, s);
+  {
+    double hp = g.h[0];
+    for (int i = 1; i < N; ++i) hp *= g.h[i];
+    const double two_fb = ldexp(1.0, frac_bits_for(n));
+    const double scaleM = (double)n * hp * two_fb;
+    const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
+    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, B, d.specD, pl->ctx->tw64, s);
+  }
+  for (int ax = 1; ax < N; ++ax)
+    launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
+                                        nullptr, nullptr, 0, pl->ctx->tw64, s);
+  const cplx<T>* spec;
+  if (sizeof(T) == 4) {
+    launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
+    spec = (const cplx<T>*)d.spec;
+  } else {
+    spec = (const cplx<T>*)d.specD;
+  }
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[1], s), "event");
+
+  // ---- ladder: log-VMR for every rung ----
+  const T* gM = (const T*)d.gM;
+  const T* gV = (const T*)d.gV;
+  if (N == 1) {
+    launch_ladder_vmr<T>(g, spec, 0, B, n, S, 0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+  } else {
+    const int64_t per_rung = 2 * (int64_t)B * fs;
+    int Rc = (int)std::max<int64_t>(1, std::min<int64_t>(S, d.Y_elems / per_rung));
+    cplx<T>* Y = (cplx<T>*)d.Y;
+    for (int r0 = 0; r0 < S; r0 += Rc) {
+      const int R = std::min(Rc, S - r0);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw, s);
+      if (N == 3)
+        launch_axis_fft<T, T, 1>(g, 2, Y, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw, s);
+      launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+    }
+  }
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
+
+  // ---- convergence + means at the chosen rung ----
+  int16_t* chosen = d_chosen ? d_chosen : d.chosen;
+  launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, s);
+  if (N == 1) {
+    launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, d.sel, d_density, s);
+  } else {
+    int Rm = (int)std::max<int64_t>(1, std::min<int64_t>(S, d.Y_elems / fs));
+    if (Rm > 1 && (Rm & 1)) --Rm;
+    cplx<T>* Y = (cplx<T>*)d.Y;
+    const int fM = 2 * B;
+    for (int r0 = 0; r0 < S; r0 += Rm) {
+      const int R = std::min(Rm, S - r0);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      if (N == 3)
+        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
+    }
+  }
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
+  return check_hip(hipGetLastError(), "kernel launch");
+}
+
+int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
+                 double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
+  if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
+  if (n >= (1ll << 31)) return fail(PKDE_INVALID_ARGUMENT, "more than 2^31-1 samples");
+  int rc = plan_reserve(pl, n, false);
+  if (rc) return rc;
+  pl->last_n = n;
+  if (pl->prec) return enqueue_typed<double>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
+  return enqueue_typed<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
+}
+
+}  // namespace pkde
+
+// ====================================================================== C ABI
+extern "C" {
+
+int pkde_version(void) { return PKDE_ABI_VERSION; }
+
+const char* pkde_last_error(void) { return pkde::g_last_error.c_str(); }
+
+void pkde_gradepro_params_default(pkde_gradepro_params* p) {
+  p->n_bootstraps = 100;
+  p->n_steps = 250;
+  p->bw_step = NAN;
+  for (int i = 0; i < PKDE_MAX_DIMS; ++i) p->bw_final[i] = NAN;
+  p->alpha_s = NAN;
+  p->alpha_os = NAN;
+  p->eps = NAN;
+  p->rng_seed = 0x5eed;
+  p->deterministic_binning = 1;
+  p->precision = 0;
+}
+
+int pkde_context_create(int device, pkde_ctx** out) {
+  if (!out) return fail(PKDE_INVALID_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(PKDE_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(PKDE_NO_DEVICE, "device index out of range");
+  PKDE_TRY(hipSetDevice(device), "hipSetDevice");
+  pkde_ctx* c = new pkde_ctx();
+  c->device = device;
+  int rc = check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  // twiddles exp(-2 pi i k / L), L = 2..4096, computed in FP64 on the host
+  const int total = 8192 - 2;
+  std::vector<double> t64(2 * (size_t)total);
+  std::vector<float> t32(2 * (size_t)total);
+  for (int lg = 1; lg <= 12; ++lg) {
+    const int L = 1 << lg;
+    for (int k = 0; k < L; ++k) {
+      const double a = 6.283185307179586476925286766559 * (double)k / (double)L;
+      const double cr = cos(a), ci = -sin(a);
+      t64[2 * (size_t)(L - 2 + k)] = cr;
+      t64[2 * (size_t)(L - 2 + k) + 1] = ci;
+      t32[2 * (size_t)(L - 2 + k)] = (float)cr;
+      t32[2 * (size_t)(L - 2 + k) + 1] = (float)ci;
+    }
+  }
+  rc = check_hip(hipMalloc((void**)&c->tw64, t64.size() * sizeof(double)), "twiddles");
+  if (!rc) rc = check_hip(hipMalloc((void**)&c->tw32, t32.size() * sizeof(float)), "twiddles");
+  if (!rc) rc = check_hip(hipMemcpy(c->tw64, t64.data(), t64.size() * 8, hipMemcpyHostToDevice), "twiddles");
+  if (!rc) rc = check_hip(hipMemcpy(c->tw32, t32.data(), t32.size() * 4, hipMemcpyHostToDevice), "twiddles");
+  if (rc) {
+    pkde_context_destroy(c);
+    return rc;
+  }
+  if (const char* e = getenv("PKDE_CHUNK_BYTES")) c->chunk_bytes = strtoull(e, nullptr, 10);
+  *out = c;
+  return PKDE_OK;
+}
+
+void pkde_context_destroy(pkde_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->tw64) (void)hipFree(ctx->tw64);
+  if (ctx->tw32) (void)hipFree(ctx->tw32);
+  delete ctx;
+}
+
+int pkde_device_free_memory(pkde_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!ctx) return fail(PKDE_INVALID_ARGUMENT, "null context");
+  PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  size_t f = 0, t = 0;
+  PKDE_TRY(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return PKDE_OK;
+}
+
+int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
+                              const double* hi, const double* h, const pkde_gradepro_params* params,
+                              pkde_plan** out) {
+  if (!ctx || !out || !n_per_axis || !lo || !hi || !h)
+    return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (ndims < 1 || ndims > PKDE_MAX_DIMS)
+    return fail(PKDE_UNSUPPORTED, "the :rocm estimator supports 1 to 3 dimensions");
+  pkde_gradepro_params p;
+  if (params)
+    p = *params;
+  else
+    pkde_gradepro_params_default(&p);
+  if (p.n_bootstraps < 2) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps must be at least 2");
+  if (p.n_bootstraps > 16383) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps too large");
+  pkde_plan* pl = new pkde_plan();
+  pl->ctx = ctx;
+  pl->p = p;
+  GridDev& g = pl->g;
+  g.N = ndims;
+  g.G = 1;
+  g.ksum = 0;
+  for (int i = 0; i < PKDE_MAX_DIMS; ++i) {
+    g.n[i] = 1;
+    g.log2n[i] = 0;
+    g.lo[i] = g.hi[i] = g.h[i] = 0.0;
+    g.koff[i] = 0;
+  }
+  for (int i = 0; i < ndims; ++i) {
+    if (!is_pow2(n_per_axis[i]) || n_per_axis[i] > 4096) {
+      delete pl;
+      return fail(PKDE_UNSUPPORTED, "grid axis sizes must be powers of two in [2, 4096]");
+    }
+    if (!(h[i] > 0) || !(hi[i] > lo[i]) || !isfinite(lo[i]) || !isfinite(hi[i])) {
+      delete pl;
+      return fail(PKDE_INVALID_ARGUMENT, "invalid grid bounds or spacing");
+    }
+    g.n[i] = (int)n_per_axis[i];
+    g.log2n[i] = ilog2(n_per_axis[i]);
+    g.lo[i] = lo[i];
+    g.hi[i] = hi[i];
+    g.h[i] = h[i];
+    g.koff[i] = g.ksum;
+    g.ksum += g.n[i];
+    g.G *= g.n[i];
+  }
+  g.H1 = g.n[0] / 2 + 1;
+  g.Q = g.G / g.n[0];
+  pl->B = p.n_bootstraps;
+  pl->prec = p.precision ? 1 : 0;
+  pl->fixed = p.deterministic_binning ? 1 : 0;
+  // defaults (GradeProEstimator.jl:541-550, 754-766)
+  pl->alpha_s = isnan(p.alpha_s) ? (ndims == 1 ? 0.0 : 0.06) : p.alpha_s;
+  pl->alpha_os = isnan(p.alpha_os) ? 0.1 : p.alpha_os;
+  pl->eps = isnan(p.eps) ? (ndims == 1 ? 2.0 : -0.5) : p.eps;
+  if (pl->alpha_s < 0 || pl->alpha_s > 1 || pl->alpha_os < 0 || pl->alpha_os > 1) {
+    delete pl;
+    return fail(PKDE_INVALID_ARGUMENT, "Fraction must be in the range [0, 1]");
+  }
+  pl->bw_given = true;
+  for (int i = 0; i < ndims; ++i) {
+    pl->bw_host[i] = p.bw_final[i];
+    pl->bw_given = pl->bw_given && isfinite(p.bw_final[i]);
+  }
+  if (pl->bw_given)
+    for (int i = 0; i < ndims; ++i)
+      if (!(pl->bw_host[i] > 0)) {
+        delete pl;
+        return fail(PKDE_INVALID_ARGUMENT, "bw_final must be positive");
+      }
+  // ladder length (GradeProEstimator.jl:860-905)
+  if (p.n_steps > 0) {
+    pl->steps_mode = 1;
+    pl->steps_or_step = (double)p.n_steps;
+    pl->S = p.n_steps + 1;
+  } else if (isfinite(p.bw_step)) {
+    if (!pl->bw_given) {
+      delete pl;
+      return fail(PKDE_UNSUPPORTED, "bw_step with the Silverman bw_final needs bw_final on :rocm");
+    }
+    double m = INFINITY;
+    for (int i = 0; i < ndims; ++i) {
+      if (p.bw_step > pl->bw_host[i] || p.bw_step <= 0.0) {
+        delete pl;
+        return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
+      }
+      m = std::min(m, floor(pl->bw_host[i] / p.bw_step));
+    }
+    pl->steps_mode = 0;
+    pl->steps_or_step = p.bw_step;
+    pl->S = (int)m + 1;
+  } else {
+    pl->steps_mode = 1;
+    pl->steps_or_step = 250.0;
+    pl->S = 251;
+  }
+  if (pl->S > 32000) {
+    delete pl;
+    return fail(PKDE_INVALID_ARGUMENT, "too many bandwidth steps");
+  }
+  int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
+  // Fourier coordinates 2pi*fftfreq(n, 1/h) (Grids.jl:225-231)
+  std::vector<double> kf(g.ksum);
+  for (int i = 0; i < ndims; ++i) {
+    const int64_t ni = g.n[i];
+    const double mult = ((1.0 / g.h[i]) / (double)ni) * 6.283185307179586;
+    const int64_t nn = (ni + 1) >> 1;
+    for (int64_t j = 0; j < ni; ++j) kf[g.koff[i] + j] = (double)(j < nn ? j : j - ni) * mult;
+  }
+  if (!rc) rc = dalloc((void**)&pl->d.kfreq, kf.size() * 8, "kfreq");
+  if (!rc) rc = check_hip(hipMemcpy(pl->d.kfreq, kf.data(), kf.size() * 8, hipMemcpyHostToDevice), "kfreq");
+  if (!rc) rc = dalloc((void**)&pl->d.bw, 256, "bw");
+  if (!rc && pl->bw_given)
+    rc = check_hip(hipMemcpy(pl->d.bw, pl->bw_host, ndims * 8, hipMemcpyHostToDevice), "bw");
+  for (int k = 0; k < 4 && !rc; ++k) rc = check_hip(hipEventCreate(&pl->ev[k]), "event");
+  if (rc) {
+    plan_free(pl);
+    delete pl;
+    return rc;
+  }
+  *out = pl;
+  return PKDE_OK;
+}
+
+void pkde_gradepro_plan_destroy(pkde_plan* plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  (void)hipStreamSynchronize(plan->ctx->stream);
+  plan_free(plan);
+  delete plan;
+}
+
+uint64_t pkde_gradepro_plan_footprint(const pkde_plan* plan, int64_t n_samples) {
+  return plan ? plan_bytes(plan, n_samples) : 0;
+}
+
+static int finish_stats(pkde_plan* pl, int64_t n, pkde_stats* st) {
+  if (!st) return PKDE_OK;
+  memset(st, 0, sizeof(*st));
+  st->n_rungs = pl->S;
+  double par[3] = {0, 0, 0};
+  PKDE_TRY(hipMemcpy(par, pl->d.params, sizeof(par), hipMemcpyDeviceToHost), "params");
+  PKDE_TRY(hipMemcpy(st->bw_final, pl->d.bw, pl->g.N * 8, hipMemcpyDeviceToHost), "bw");
+  st->eps = par[0];
+  st->steps_low = (int32_t)par[1];
+  st->steps_over = (int32_t)par[2];
+  float ms = 0;
+  PKDE_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[3]), "event");
+  st->ms_total = ms;
+  PKDE_TRY(hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]), "event");
+  st->ms_ladder = ms;
+  const double G = (double)pl->g.G, U = G * pl->S, B = pl->B;
+  st->algorithmic_bytes = U * (8.0 * B + 4.0) + (double)n * pl->g.N * 8.0 + (2 * B + 1) * G * 4.0 + G * 6.0;
+  return PKDE_OK;
+}
+
+int pkde_gradepro_estimate(pkde_plan* pl, const double* samples, int64_t n, const int32_t* bootstrap_idx,
+                           double* density_out, int16_t* chosen_rung_out, pkde_stats* stats) {
+  if (!pl || !samples || !density_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
+  std::lock_guard<std::mutex> lk(pl->ctx->mu);
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  hipStream_t s = pl->ctx->stream;
+  int rc = plan_reserve(pl, n, bootstrap_idx != nullptr);
+  if (rc) return rc;
+  PKDE_TRY(hipMemcpyAsync(pl->d.x, samples, (size_t)n * pl->g.N * 8, hipMemcpyHostToDevice, s), "upload samples");
+  if (bootstrap_idx)
+    PKDE_TRY(hipMemcpyAsync(pl->d.idx, bootstrap_idx, (size_t)n * pl->B * 4, hipMemcpyHostToDevice, s),
+             "upload indices");
+  rc = plan_enqueue(pl, pl->d.x, n, bootstrap_idx ? pl->d.idx : nullptr, pl->d.density, pl->d.chosen, s, true);
+  if (rc) return rc;
+  const int64_t G = pl->g.G;
+  PKDE_TRY(hipMemcpyAsync(density_out, pl->d.density, (size_t)G * 8, hipMemcpyDeviceToHost, s), "download density");
+  if (chosen_rung_out)
+    PKDE_TRY(hipMemcpyAsync(chosen_rung_out, pl->d.chosen, (size_t)G * 2, hipMemcpyDeviceToHost, s), "download chosen");
+  int err = 0;
+  PKDE_TRY(hipMemcpyAsync(&err, pl->d.err, 4, hipMemcpyDeviceToHost, s), "download flag");
+  PKDE_TRY(hipStreamSynchronize(s), "estimate");
+  if (err & 1) return fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid. Remove them or increase grid boundaries.");
+  if (err & 2) return fail(PKDE_INVALID_ARGUMENT, "bootstrap index outside 0..n-1");
+  return finish_stats(pl, n, stats);
+}
+
+int pkde_gradepro_estimate_device(pkde_plan* pl, const double* d_samples, int64_t n,
+                                  const int32_t* d_bootstrap_idx, double* d_density,
+                                  int16_t* d_chosen_rung, void* stream) {
+  if (!pl || !d_samples || !d_density) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  hipStream_t s = stream ? (hipStream_t)stream : pl->ctx->stream;
+  return plan_enqueue(pl, d_samples, n, d_bootstrap_idx, d_density, d_chosen_rung, s, true);
+}
+
+int pkde_gradepro_check(pkde_plan* pl) {
+  if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
+  int err = 0;
+  PKDE_TRY(hipMemcpy(&err, pl->d.err, 4, hipMemcpyDeviceToHost), "flag");
+  if (err & 1) return fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid. Remove them or increase grid boundaries.");
+  if (err & 2) return fail(PKDE_INVALID_ARGUMENT, "bootstrap index outside 0..n-1");
+  return PKDE_OK;
+}
+
+int pkde_gradepro_last_stats(pkde_plan* pl, pkde_stats* stats) {
+  if (!pl || !stats) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
+  return finish_stats(pl, pl->last_n, stats);
+}
+
+int pkde_gradepro_debug_enable(pkde_plan* pl, int enable) {
+  if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  pl->debug = enable != 0;
+  return PKDE_OK;
+}
+
+int pkde_gradepro_debug_vmr(pkde_plan* pl, double* vmr_out, int64_t capacity) {
+  if (!pl || !vmr_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  const int64_t need = (int64_t)pl->S * pl->g.G;
+  if (capacity < need) return fail(PKDE_INVALID_ARGUMENT, "vmr buffer too small");
+  if (!pl->d.vmr) return fail(PKDE_INVALID_ARGUMENT, "no estimate has run");
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  PKDE_TRY(hipStreamSynchronize(pl->ctx->stream), "sync");
+  PKDE_TRY(hipMemcpy(vmr_out, pl->d.vmr, (size_t)need * 8, hipMemcpyDeviceToHost), "vmr");
+  return PKDE_OK;
+}
+
+int pkde_gradepro_convergence(pkde_ctx* ctx, int64_t G, int32_t S, const double* vmr, const double* dlogt,
+                              double eps, int32_t steps_low, int32_t steps_over, int16_t* chosen_out) {
+  if (!ctx || !vmr || !dlogt || !chosen_out || G < 1 || S < 1)
+    return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  double *dv = nullptr, *dd = nullptr, *dp = nullptr;
+  int16_t *dc = nullptr, *ds = nullptr;
+  int rc = dalloc((void**)&dv, (size_t)S * G * 8, "vmr");
+  if (!rc) rc = dalloc((void**)&dd, (size_t)S * 8, "dlogt");
+  if (!rc) rc = dalloc((void**)&dp, 256, "params");
+  if (!rc) rc = dalloc((void**)&dc, (size_t)G * 2, "chosen");
+  if (!rc) rc = dalloc((void**)&ds, (size_t)G * 2, "sel");
+  const double par[3] = {eps, (double)steps_low, (double)steps_over};
+  if (!rc) rc = check_hip(hipMemcpy(dv, vmr, (size_t)S * G * 8, hipMemcpyHostToDevice), "upload");
+  if (!rc) rc = check_hip(hipMemcpy(dd, dlogt, (size_t)S * 8, hipMemcpyHostToDevice), "upload");
+  if (!rc) rc = check_hip(hipMemcpy(dp, par, sizeof(par), hipMemcpyHostToDevice), "upload");
+  if (!rc) {
+    launch_convergence(G, S, dv, dd, dp, dc, ds, ctx->stream);
+    rc = check_hip(hipStreamSynchronize(ctx->stream), "convergence");
+  }
+  if (!rc) rc = check_hip(hipMemcpy(chosen_out, dc, (size_t)G * 2, hipMemcpyDeviceToHost), "download");
+  dfree(dv); dfree(dd); dfree(dp); dfree(dc); dfree(ds);
+  return rc;
+}
+
+int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
+                         const double* hi, const double* h, const double* samples, int64_t n,
+                         const int32_t* bootstrap_idx, int32_t B, int deterministic, double* sums_out,
+                         double* squares_out) {
+  if (!ctx || !n_per_axis || !samples || !bootstrap_idx || !sums_out || B < 1 || n < 1)
+    return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
+  if (ndims < 1 || ndims > PKDE_MAX_DIMS) return fail(PKDE_UNSUPPORTED, "1 to 3 dimensions");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  GridDev g{};
+  g.N = ndims;
+  g.G = 1;
+  for (int i = 0; i < PKDE_MAX_DIMS; ++i) g.n[i] = 1;
+  for (int i = 0; i < ndims; ++i) {
+    if (n_per_axis[i] < 2 || !(h[i] > 0)) return fail(PKDE_INVALID_ARGUMENT, "invalid grid");
+    g.n[i] = (int)n_per_axis[i];
+    g.lo[i] = lo[i];
+    g.hi[i] = hi[i];
+    g.h[i] = h[i];
+    g.G *= g.n[i];
+  }
+  const int64_t G = g.G, F = 2 * (int64_t)B + 1;
+  double* dx = nullptr;
+  int32_t* di = nullptr;
+  void* acc = nullptr;
+  int* derr = nullptr;
+  int rc = dalloc((void**)&dx, (size_t)n * ndims * 8, "samples");
+  if (!rc) rc = dalloc((void**)&di, (size_t)n * B * 4, "indices");
+  if (!rc) rc = dalloc(&acc, (size_t)F * G * 8, "acc");
+  if (!rc) rc = dalloc((void**)&derr, 256, "err");
+  if (!rc) rc = check_hip(hipMemcpy(dx, samples, (size_t)n * ndims * 8, hipMemcpyHostToDevice), "upload");
+  if (!rc) rc = check_hip(hipMemcpy(di, bootstrap_idx, (size_t)n * B * 4, hipMemcpyHostToDevice), "upload");
+  int err = 0;
+  if (!rc) {
+    hipStream_t s = ctx->stream;
+    launch_fill_i32(derr, 0, 1, s);
+    launch_zero_bytes(acc, F * G * 8, s);
+    const int fb = frac_bits_for(n);
+    launch_binning(g, dx, n, di, B, 0, deterministic, fb, acc, derr, s);
+    rc = check_hip(hipStreamSynchronize(s), "binning");
+    if (!rc) rc = check_hip(hipMemcpy(&err, derr, 4, hipMemcpyDeviceToHost), "flag");
+    if (!rc) {
+      std::vector<double> host((size_t)F * G);
+      rc = check_hip(hipMemcpy(host.data(), acc, (size_t)F * G * 8, hipMemcpyDeviceToHost), "download");
+      if (!rc) {
+        double hp = g.h[0];
+        for (int i = 1; i < ndims; ++i) hp *= g.h[i];
+        const double two_fb = ldexp(1.0, fb);
+        const double scM = (double)n * hp * two_fb;
+        const double scV = ((double)n * hp) * ((double)n * hp) * two_fb;
+        const long long* q = (const long long*)host.data();
+        for (int64_t i = 0; i < (int64_t)B * G; ++i) {
+          sums_out[i] = deterministic ? (double)q[i] / scM : host[i];
+          if (squares_out)
+            squares_out[i] = deterministic ? (double)q[(int64_t)B * G + i] / scV : host[(int64_t)B * G + i];
+        }
+      }
+    }
+  }
+  dfree(dx); dfree(di); dfree(acc); dfree(derr);
+  if (rc) return rc;
+  if (err & 1) return fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid. Remove them or increase grid boundaries.");
+  if (err & 2) return fail(PKDE_INVALID_ARGUMENT, "bootstrap index outside 0..n-1");
+  return PKDE_OK;
+}
+
+}  // extern "C"

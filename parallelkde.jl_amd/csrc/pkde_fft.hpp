@@ -1,0 +1,203 @@
+// Device-side building blocks for the :gradepro HIP path on gfx950:
+// complex arithmetic, in-register radix-2/4/8/16 DFTs and a batched Stockham
+// FFT whose passes exchange data through LDS.  Pencils of power-of-two
+// length L <= 4096 live in LDS; every thread owns E = min(L, 16) elements per
+// pass, so a pass is one LDS read, the radix-R butterflies in registers, and
+// one LDS write.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pkde {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return {a.re * s, a.im * s}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return {a.re, -a.im}; }
+
+// cos / sin of 2*pi*k/16, k = 0..15 (exact decimal expansions of the FP64 values)
+__device__ constexpr double kCos16[16] = {
+    1.0, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978,
+    0.0, -0.38268343236508978, -0.70710678118654757, -0.92387953251128674,
+    -1.0, -0.92387953251128674, -0.70710678118654757, -0.38268343236508978,
+    0.0, 0.38268343236508978, 0.70710678118654757, 0.92387953251128674};
+__device__ constexpr double kSin16[16] = {
+    0.0, 0.38268343236508978, 0.70710678118654757, 0.92387953251128674,
+    1.0, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978,
+    0.0, -0.38268343236508978, -0.70710678118654757, -0.92387953251128674,
+    -1.0, -0.92387953251128674, -0.70710678118654757, -0.38268343236508978};
+
+// x * exp(DIR * 2 pi i k / R) for compile-time R <= 16; k resolves to a
+// constant after unrolling, so the 0 and quarter-turn cases cost nothing.
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cplx<T> twiddle_const(cplx<T> x, int k) {
+  const int j = (k * (16 / R)) & 15;
+  if (j == 0) return x;
+  if (j == 4) return DIR > 0 ? cplx<T>{-x.im, x.re} : cplx<T>{x.im, -x.re};
+  if (j == 8) return {-x.re, -x.im};
+  if (j == 12) return DIR > 0 ? cplx<T>{x.im, -x.re} : cplx<T>{-x.im, x.re};
+  const T c = (T)kCos16[j];
+  const T s = (T)(DIR * kSin16[j]);
+  return {x.re * c - x.im * s, x.re * s + x.im * c};
+}
+
+// In-register DFT of R = 2^m points, v[r] -> V[s] = sum_r v[r] exp(DIR 2pi i r s / R).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft_reg(cplx<T>* v) {
+  if constexpr (R == 1) {
+    return;
+  } else if constexpr (R == 2) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else {
+    constexpr int H = R / 2;
+    cplx<T> e[H], o[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    dft_reg<H, DIR, T>(e);
+    dft_reg<H, DIR, T>(o);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      cplx<T> t = twiddle_const<R, DIR, T>(o[k], k);
+      v[k] = cadd(e[k], t);
+      v[k + H] = csub(e[k], t);
+    }
+  }
+}
+
+// Radix plan for L = 2^log2L: passes of 16 and one trailing 8/4/2.
+struct FftPlanDev {
+  int L, log2L, E, tpp;  // E: elements per thread, tpp: threads per pencil
+};
+
+__device__ __forceinline__ FftPlanDev make_plan(int log2L) {
+  FftPlanDev p;
+  p.log2L = log2L;
+  p.L = 1 << log2L;
+  p.E = p.L < 16 ? p.L : 16;
+  p.tpp = p.L / p.E;
+  return p;
+}
+
+// One Stockham pass of radix R over `npen` pencils of length L stored at
+// buf[p * pitch + e].  Each work item is a (pencil, thread-in-pencil) pair that
+// performs E/R butterflies; items are processed in rounds of nthr so every
+// thread meets the same barriers.
+// tw[k] = exp(-2 pi i k / L), k in [0, L) (FP64-accurate, rounded to T).
+template <int R, int E, int DIR, typename T>
+__device__ __forceinline__ void stockham_pass(cplx<T>* buf, int L, int npen, int pitch, int Ns,
+                                              const cplx<T>* __restrict__ tw, int tid, int nthr) {
+  const int tpp = L / E;
+  const int items = npen * tpp;
+  const int nbfly = L / R;  // butterflies per pencil
+  const int twstep = L / (Ns * R);
+  for (int base = 0; base < items; base += nthr) {
+    const int item = base + tid;
+    const bool active = item < items;
+    cplx<T> v[E];
+    int pen = 0, t = 0;
+    if (active) {
+      pen = item / tpp;
+      t = item - pen * tpp;
+      const cplx<T>* pb = buf + pen * pitch;
+#pragma unroll
+      for (int u = 0; u < E / R; ++u) {
+        const int j = t + u * tpp;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u * R + r] = pb[j + r * nbfly];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      cplx<T>* pb = buf + pen * pitch;
+#pragma unroll
+      for (int u = 0; u < E / R; ++u) {
+        const int j = t + u * tpp;
+        const int jm = j % Ns;
+        cplx<T>* w = v + u * R;
+        if (jm != 0) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            cplx<T> f = tw[(jm * r * twstep) & (L - 1)];
+            if (DIR > 0) f.im = -f.im;
+            w[r] = cmul(w[r], f);
+          }
+        }
+        dft_reg<R, DIR, T>(w);
+        const int out0 = (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) pb[out0 + r * Ns] = w[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Batched in-LDS FFT: npen pencils of length L = 2^log2L (L <= 4096) at
+// buf[p * pitch + e], transformed in place (natural order in and out).
+// Unnormalised; DIR = -1 forward, +1 inverse.  Must be called by the whole
+// workgroup (contains barriers).
+template <int DIR, typename T>
+__device__ void lds_fft(cplx<T>* buf, int log2L, int npen, int pitch,
+                        const cplx<T>* __restrict__ tw, int tid, int nthr) {
+  const int L = 1 << log2L;
+  if (L == 1) return;
+  int Ns = 1;
+  int rem = log2L;
+  if (L >= 16) {
+    while (rem >= 4) {
+      stockham_pass<16, 16, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+      Ns *= 16;
+      rem -= 4;
+    }
+    if (rem == 3) stockham_pass<8, 16, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+    else if (rem == 2) stockham_pass<4, 16, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+    else if (rem == 1) stockham_pass<2, 16, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+  } else if (L == 8) {
+    stockham_pass<8, 8, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+  } else if (L == 4) {
+    stockham_pass<4, 4, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+  } else if (L == 2) {
+    stockham_pass<2, 2, DIR, T>(buf, L, npen, pitch, Ns, tw, tid, nthr);
+  }
+}
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11): counter-based bootstrap draws.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint32_t)p1;
+    c[2] = n2;
+    c[3] = (uint32_t)p0;
+  }
+}
+
+}  // namespace pkde

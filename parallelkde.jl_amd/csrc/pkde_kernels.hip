@@ -1,0 +1,903 @@
+// HIP kernels of the :gradepro path for gfx950.
+//
+// Data flow of one estimate (ParallelKDE.jl GradeProEstimator.jl:700-1072):
+//   samples (FP64)
+//     -> k_binning         linear binning of B resamples + full sample
+//                          (DirectSpace.jl:196-242), exact FP64 cell index,
+//                          int64 fixed-point or FP64 atomic deposits
+//     -> k_forward_axis1   FP64 forward DFT along axis 1, two real fields per
+//                          complex transform, Hermitian halves kept
+//     -> k_axis_fft        FP64 forward DFT along axes 2..N
+//     -> spectra [field][k1][q] in FP32 (or FP64), resident for all rungs
+//   per rung chunk:
+//     -> k_axis_fft        Gaussian multiply + inverse DFT along axes 2..N
+//     -> k_ladder_vmr      Gaussian multiply + inverse DFT along axis 1 of two
+//                          bootstraps per complex transform, then the VMR and
+//                          an FP64 Welford over bootstraps (calculate_scaled_vmr!)
+//   -> k_convergence       per-point state machine over all rungs (find_stability)
+//   -> k_means_select      full-sample means at each point's chosen rung
+//
+// Rungs are independent until the state machine, so every rung of a chunk is
+// processed in parallel; the state machine only needs the per-rung log-VMR.
+#include "pkde_kernels.hpp"
+
+#include <math.h>
+
+namespace pkde {
+
+// ------------------------------------------------------------------ utilities
+__global__ void k_fill_i32(int* p, int v, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+static inline int grid_for(int64_t work, int threads = kThreads, int cap = 65536) {
+  int64_t b = (work + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+void launch_fill_i32(int* p, int v, int64_t count, hipStream_t s) {
+  k_fill_i32<<<grid_for(count), kThreads, 0, s>>>(p, v, count);
+}
+
+void launch_zero_bytes(void* p, int64_t bytes, hipStream_t s) {
+  // all buffers are allocated with 256-byte granularity
+  int64_t words = (bytes + 7) / 8;
+  k_zero_u64<<<grid_for(words, kThreads, 1 << 16), kThreads, 0, s>>>((unsigned long long*)p, words);
+}
+
+// -------------------------------------------------------------------- binning
+// One thread handles four consecutive draws of one replicate (one Philox
+// block).  Replicate index B is the full sample (identity draws); only those
+// threads raise the out-of-grid flag, so every sample is checked exactly once
+// (check_data, DirectSpace.jl:479-498).  Cell index and corner weights use the
+// same FP64 operations in the same order as generate_dirac_cpu!
+// (DirectSpace.jl:215-237): floor(delta/h), rem(delta, h), h - rem, and the
+// corner weight prod(low weights) * prod(high weights) with axis 1 as the most
+// significant mask bit.
+template <bool FIXED>
+__global__ __launch_bounds__(kThreads) void k_binning(GridDev g, const double* __restrict__ x,
+                                                       int64_t n, const int32_t* __restrict__ idx,
+                                                       int B, uint32_t key0, uint32_t key1,
+                                                       double denom, double scaleM, double scaleV,
+                                                       void* __restrict__ acc, int* err) {
+  const int N = g.N;
+  const int64_t nq = (n + 3) >> 2;
+  const int64_t total = (int64_t)(B + 1) * nq;
+  int64_t stride[kMaxDims];
+  stride[0] = 1;
+  for (int i = 1; i < N; ++i) stride[i] = stride[i - 1] * g.n[i - 1];
+
+  for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(it / nq);
+    const int64_t blk = it - (int64_t)b * nq;
+    const int64_t s0 = blk * 4;
+    int64_t src[4];
+    bool src_ok[4] = {true, true, true, true};
+    if (b == B) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) src[q] = s0 + q;
+    } else if (idx) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int64_t v = (s0 + q < n) ? (int64_t)idx[(int64_t)b * n + s0 + q] : 0;
+        src_ok[q] = (v >= 0 && v < n);
+        src[q] = src_ok[q] ? v : 0;
+      }
+    } else {
+      uint32_t c[4] = {(uint32_t)blk, (uint32_t)((uint64_t)blk >> 32), (uint32_t)b, 0u};
+      philox4x32_10(c, key0, key1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) src[q] = (int64_t)(((uint64_t)c[q] * (uint64_t)n) >> 32);
+    }
+    for (int q = 0; q < 4; ++q) {
+      if (s0 + q >= n) break;
+      if (!src_ok[q]) {
+        atomicOr(err, 2);
+        continue;
+      }
+      const double* xs = x + src[q] * N;
+      int64_t il[kMaxDims];
+      double rl[kMaxDims], rh[kMaxDims];
+      bool ok = true;
+      for (int i = 0; i < N; ++i) {
+        const double xi = xs[i];
+        const double delta = xi - g.lo[i];
+        const double l = floor(delta / g.h[i]);
+        rh[i] = fmod(delta, g.h[i]);
+        rl[i] = g.h[i] - rh[i];
+        ok = ok && (xi > g.lo[i]) && (xi < g.hi[i]) && (l <= (double)(g.n[i] - 2));
+        il[i] = ok ? (int64_t)l : 0;
+      }
+      if (!ok) {
+        if (b == B) atomicOr(err, 1);
+        continue;
+      }
+      for (int c = 0; c < (1 << N); ++c) {
+        int64_t off = 0;
+        double pl = 1.0, ph = 1.0;
+        bool hl = false, hh = false;
+        for (int j = 0; j < N; ++j) {
+          const int mask = (c >> (N - 1 - j)) & 1;
+          off += (mask ? il[j] : il[j] + 1) * stride[j];
+          if (mask) {
+            pl = hl ? pl * rl[j] : rl[j];
+            hl = true;
+          } else {
+            ph = hh ? ph * rh[j] : rh[j];
+            hh = true;
+          }
+        }
+        const double term = (pl * ph) / denom;
+        const int64_t fM = (b == B) ? (int64_t)2 * B : (int64_t)b;
+        if (FIXED) {
+          unsigned long long* a = (unsigned long long*)acc;
+          atomicAdd(a + fM * g.G + off, (unsigned long long)llrint(term * scaleM));
+          if (b < B)
+            atomicAdd(a + ((int64_t)B + b) * g.G + off, (unsigned long long)llrint((term * term) * scaleV));
+        } else {
+          double* a = (double*)acc;
+          atomicAdd(a + fM * g.G + off, term);
+          if (b < B) atomicAdd(a + ((int64_t)B + b) * g.G + off, term * term);
+        }
+      }
+    }
+  }
+}
+
+void launch_binning(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
+                    uint64_t seed, int fixed, int frac_bits, void* acc, int* err,
+                    hipStream_t s) {
+  double hp = g.h[0];
+  for (int i = 1; i < g.N; ++i) hp *= g.h[i];
+  const double spacing_squared = hp * hp;
+  const double denom = (double)n * spacing_squared;
+  const double two_fb = ldexp(1.0, frac_bits);
+  const double scaleM = (double)n * hp * two_fb;
+  const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
+  const int64_t work = (int64_t)(B + 1) * ((n + 3) >> 2);
+  const int blocks = grid_for(work, kThreads, 1 << 20);
+  if (fixed)
+    k_binning<true><<<blocks, kThreads, 0, s>>>(g, x, n, idx, B, (uint32_t)seed,
+                                                (uint32_t)(seed >> 32), denom, scaleM, scaleV,
+                                                acc, err);
+  else
+    k_binning<false><<<blocks, kThreads, 0, s>>>(g, x, n, idx, B, (uint32_t)seed,
+                                                 (uint32_t)(seed >> 32), denom, scaleM, scaleV,
+                                                 acc, err);
+}
+
+// ------------------------------------------------------ forward DFT, axis 1
+// Real fields are paired within their group (sums with sums, squares with
+// squares) so both members have the same magnitude; z = a + i b is
+// transformed once and split with the Hermitian symmetry of real inputs.
+__device__ __forceinline__ void pair_fields(int pi, int B, int& fa, int& fb) {
+  const int nMp = (B + 2) / 2;  // sums + full sample: B + 1 fields
+  if (pi < nMp) {
+    const int a = 2 * pi, c = 2 * pi + 1;
+    fa = (a < B) ? a : 2 * B;
+    fb = (c < B) ? c : (c == B ? 2 * B : -1);
+  } else {
+    const int pv = pi - nMp;
+    fa = B + 2 * pv;
+    fb = (2 * pv + 1 < B) ? fa + 1 : -1;
+  }
+}
+
+template <bool FIXED>
+__device__ __forceinline__ double acc_value(const void* acc, int f, int B, int64_t G, int64_t off,
+                                            double invM, double invV) {
+  if (FIXED) {
+    const double v = (double)((const long long*)acc)[(int64_t)f * G + off];
+    return v * ((f >= B && f < 2 * B) ? invV : invM);
+  }
+  return ((const double*)acc)[(int64_t)f * G + off];
+}
+
+template <bool FIXED>
+__global__ __launch_bounds__(kThreads) void k_forward_axis1(GridDev g, const void* __restrict__ acc,
+                                                             double invM, double invV, int B, int TQ,
+                                                             cplx<double>* __restrict__ specD,
+                                                             const cplx<double>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* z = (cplx<double>*)smem;
+  const int n1 = g.n[0], H1 = g.H1;
+  const int64_t Q = g.Q;
+  const int ntile = (int)(Q / TQ);
+  const int pi = blockIdx.x / ntile;
+  const int64_t q0 = (int64_t)(blockIdx.x % ntile) * TQ;
+  int fa, fb;
+  pair_fields(pi, B, fa, fb);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int id = tid; id < TQ * n1; id += nthr) {
+    const int t = id / n1, j = id - t * n1;
+    const int64_t off = (q0 + t) * n1 + j;
+    const double va = acc_value<FIXED>(acc, fa, B, g.G, off, invM, invV);
+    const double vb = fb >= 0 ? acc_value<FIXED>(acc, fb, B, g.G, off, invM, invV) : 0.0;
+    z[t * n1 + j] = {va, vb};
+  }
+  __syncthreads();
+  lds_fft<-1, double>(z, g.log2n[0], TQ, n1, tw + (n1 - 2), tid, nthr);
+  for (int id = tid; id < TQ * H1; id += nthr) {
+    const int k = id / TQ, t = id - k * TQ;
+    const cplx<double> Zk = z[t * n1 + k];
+    const cplx<double> Zm = z[t * n1 + ((n1 - k) & (n1 - 1))];
+    const cplx<double> A = {0.5 * (Zk.re + Zm.re), 0.5 * (Zk.im - Zm.im)};
+    const cplx<double> Bv = {0.5 * (Zk.im + Zm.im), -0.5 * (Zk.re - Zm.re)};
+    const int64_t o = (int64_t)k * Q + q0 + t;
+    specD[(int64_t)fa * H1 * Q + o] = A;
+    if (fb >= 0) specD[(int64_t)fb * H1 * Q + o] = Bv;
+  }
+}
+
+void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double scaleM,
+                          double scaleV, int B, cplx<double>* specD, const double* tw64,
+                          hipStream_t s) {
+  const int n1 = g.n[0];
+  int TQ = 2048 / n1;
+  if (TQ < 1) TQ = 1;
+  if (TQ > g.Q) TQ = (int)g.Q;
+  const int npairs = (B + 2) / 2 + (B + 1) / 2;
+  const int blocks = npairs * (int)(g.Q / TQ);
+  const size_t lds = (size_t)TQ * n1 * sizeof(cplx<double>);
+  const cplx<double>* tw = (const cplx<double>*)tw64;
+  if (fixed)
+    k_forward_axis1<true><<<blocks, kThreads, lds, s>>>(g, acc, 1.0 / scaleM, 1.0 / scaleV, B, TQ,
+                                                        specD, tw);
+  else
+    k_forward_axis1<false><<<blocks, kThreads, lds, s>>>(g, acc, 1.0, 1.0, B, TQ, specD, tw);
+}
+
+// ------------------------------------------------------------- axis passes
+// Pencils along axis a (a = 1 or 2, i.e. grid axis 2 or 3) of fields laid out
+// [k1][q] with q = j2 + n2*j3.  Axis 2 pencils are contiguous; axis 3 pencils
+// have stride n2 and consecutive pencils are adjacent in memory.
+template <typename T>
+struct AxisArgs {
+  const cplx<T>* src;
+  cplx<T>* dst;
+  int64_t src_rs, dst_rs;  // rung strides (elements)
+  int64_t fsize;           // H1 * Q
+  int fid0, nf, src_fid0, dst_fid0;
+  int R, r0;
+  const T* gM;
+  const T* gV;
+  int fsplit, ksum, goff;
+  int L, log2L;
+  int64_t es, NI, OS, PF;
+  int PW, tiles_per_field;
+  const cplx<T>* tw;
+};
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = (cplx<T>*)smem;
+  const int per_rung = a.nf * a.tiles_per_field;
+  const int rr = blockIdx.x / per_rung;
+  const int rem = blockIdx.x - rr * per_rung;
+  const int fl = rem / a.tiles_per_field;
+  const int tile = rem - fl * a.tiles_per_field;
+  const int f = a.fid0 + fl;
+  const int64_t p0 = (int64_t)tile * a.PW;
+  const int npen = (int)((a.PF - p0) < a.PW ? (a.PF - p0) : a.PW);
+  const T* gt = nullptr;
+  if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize;
+  cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.fsize;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int L = a.L;
+  const int total = npen * L;
+  for (int id = tid; id < total; id += nthr) {
+    int pw, e;
+    if (a.NI == 1) {
+      pw = id / L;
+      e = id - pw * L;
+    } else {
+      e = id / npen;
+      pw = id - e * npen;
+    }
+    const int64_t p = p0 + pw;
+    const int64_t po = p / a.NI, pin = p - po * a.NI;
+    const int64_t addr = po * a.OS + pin + (int64_t)e * a.es;
+    cplx<T> v = src[addr];
+    if (gt) v = cscale(v, gt[e]);
+    buf[pw * L + e] = v;
+  }
+  __syncthreads();
+  lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
+  for (int id = tid; id < total; id += nthr) {
+    int pw, e;
+    if (a.NI == 1) {
+      pw = id / L;
+      e = id - pw * L;
+    } else {
+      e = id / npen;
+      pw = id - e * npen;
+    }
+    const int64_t p = p0 + pw;
+    const int64_t po = p / a.NI, pin = p - po * a.NI;
+    const int64_t addr = po * a.OS + pin + (int64_t)e * a.es;
+    dst[addr] = buf[pw * L + e];
+  }
+}
+
+template <typename TI, typename TO, int DIR>
+void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
+                     int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
+                     int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
+                     hipStream_t s) {
+  static_assert(sizeof(TI) == sizeof(TO), "axis passes keep the element type");
+  AxisArgs<TI> a;
+  a.src = src;
+  a.dst = (cplx<TI>*)dst;
+  a.src_rs = src_rung_stride;
+  a.dst_rs = dst_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.fid0 = fid0;
+  a.nf = nf;
+  a.src_fid0 = src_fid0;
+  a.dst_fid0 = dst_fid0;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.fsplit = fsplit;
+  a.ksum = g.ksum;
+  a.goff = g.koff[axis];
+  a.L = g.n[axis];
+  a.log2L = g.log2n[axis];
+  const int64_t n2 = g.n[1];
+  if (axis == 1) {
+    a.es = 1;
+    a.NI = 1;
+    a.OS = n2;
+  } else {
+    a.es = n2;
+    a.NI = n2;
+    a.OS = g.Q;
+  }
+  a.PF = a.fsize / a.L;
+  int PW = 4096 / a.L;
+  if (PW < 1) PW = 1;
+  if (PW > a.PF) PW = (int)a.PF;
+  a.PW = PW;
+  a.tiles_per_field = (int)((a.PF + PW - 1) / PW);
+  a.tw = (const cplx<TI>*)tw;
+  const int64_t blocks = (int64_t)R * nf * a.tiles_per_field;
+  const size_t lds = (size_t)PW * a.L * sizeof(cplx<TI>);
+  k_axis_fft<TI, DIR><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+}
+
+#define PKDE_AXIS_INST(TI, TO, DIR)                                                            \
+  template void launch_axis_fft<TI, TO, DIR>(const GridDev&, int, const cplx<TI>*, int64_t, int,  \
+                                             cplx<TO>*, int64_t, int, int, int, int, int,        \
+                                             const TI*, const TI*, int, const TI*, hipStream_t);
+PKDE_AXIS_INST(double, double, -1)
+PKDE_AXIS_INST(double, double, 1)
+PKDE_AXIS_INST(float, float, 1)
+#undef PKDE_AXIS_INST
+
+template <typename TI, typename TO>
+__global__ void k_convert(const cplx<TI>* __restrict__ src, cplx<TO>* __restrict__ dst, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const cplx<TI> v = src[i];
+    dst[i] = {(TO)v.re, (TO)v.im};
+  }
+}
+
+template <typename TI, typename TO>
+void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream_t s) {
+  k_convert<TI, TO><<<grid_for(count, kThreads, 1 << 16), kThreads, 0, s>>>(src, dst, count);
+}
+template void launch_convert<double, float>(const cplx<double>*, cplx<float>*, int64_t, hipStream_t);
+
+// ----------------------------------------------- Hermitian pair construction
+// Z(k) = A(k) + i B(k) over the full axis-1 frequency range from the halves
+// k in [0, n1/2]; the transform of Z then holds a in its real part and b in
+// its imaginary part for real fields a, b.
+template <typename T>
+__device__ __forceinline__ void put_pair(cplx<T>* row, int n1, int k1, cplx<T> A, cplx<T> Bv) {
+  row[k1] = {A.re - Bv.im, A.im + Bv.re};
+  if (k1 > 0 && 2 * k1 < n1) row[n1 - k1] = {A.re + Bv.im, Bv.re - A.im};
+}
+
+// --------------------------------------------------------------- ladder VMR
+// One workgroup owns TQ consecutive axis-1 rows (TQ*n1 grid points) of one
+// rung and walks all bootstraps in order, two at a time per complex
+// transform, NBP bootstrap pairs per LDS batch.  Per point it forms
+//   mu = |m|/n, v = |s|/n - mu^2, r = v/mu  (= |s|/|m| - |m|/n)
+// with m, s the propagated sums / squares fields (FourierSpace.jl:215-241,
+// normalised inverse FFT) and accumulates an FP64 Welford mean/M2 over the
+// bootstrap index (DirectSpace.jl:567-581).
+template <typename T>
+struct VmrArgs {
+  const cplx<T>* src;
+  int64_t src_rs, fsize;
+  int B;
+  double invn, invG;
+  int R, r0;
+  const T* gM;
+  const T* gV;
+  int ksum;
+  const double* cV;
+  const double* vscale;
+  const cplx<T>* tw;
+  double* vmr;
+  int64_t G, Q;
+  int n1, log2n1, H1, TQ, NBP, ntiles;
+};
+
+template <typename T, int PPT>
+__global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* Z = (cplx<T>*)smem;
+  const int rr = blockIdx.x / a.ntiles;
+  const int tile = blockIdx.x - rr * a.ntiles;
+  const int r = a.r0 + rr;
+  const int64_t q0 = (int64_t)tile * a.TQ;
+  const int n1 = a.n1, H1 = a.H1, TQ = a.TQ, NBP = a.NBP, B = a.B;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const T* gm1 = a.gM + (int64_t)r * a.ksum;
+  const T* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  const int npts = TQ * n1;
+  const int npen = 2 * NBP * TQ;
+
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+
+  for (int bp = 0; bp < B; bp += 2 * NBP) {
+    const int nload = NBP * 2 * H1 * TQ;
+    for (int id = tid; id < nload; id += nthr) {
+      const int t = id % TQ;
+      int rest = id / TQ;
+      const int k1 = rest % H1;
+      rest /= H1;
+      const int ty = rest & 1;
+      const int slot = rest >> 1;
+      const int b = bp + 2 * slot;
+      cplx<T>* row = Z + (int64_t)((slot * 2 + ty) * TQ + t) * n1;
+      if (b >= B) continue;
+      const int fa = ty * B + b;
+      const T gk = ty ? gv1[k1] : gm1[k1];
+      const int64_t o = (int64_t)k1 * a.Q + q0 + t;
+      cplx<T> A = cscale(src[(int64_t)fa * a.fsize + o], gk);
+      cplx<T> Bv = {(T)0, (T)0};
+      if (b + 1 < B) Bv = cscale(src[(int64_t)(fa + 1) * a.fsize + o], gk);
+      put_pair(row, n1, k1, A, Bv);
+    }
+    __syncthreads();
+    lds_fft<1, T>(Z, a.log2n1, npen, n1, a.tw + (n1 - 2), tid, nthr);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int w = tid + u * nthr;
+      if (w < npts) {
+        const int t = w / n1, j1 = w - t * n1;
+        for (int slot = 0; slot < NBP; ++slot) {
+          const int b = bp + 2 * slot;
+          if (b >= B) break;
+          const cplx<T> zm = Z[(int64_t)((slot * 2) * TQ + t) * n1 + j1];
+          const cplx<T> zv = Z[(int64_t)((slot * 2 + 1) * TQ + t) * n1 + j1];
+          {
+            const double am = fabs((double)zm.re) * a.invG;
+            const double av = fabs((double)zv.re) * vfac;
+            const double xv = av / am - am * a.invn;
+            const double d = xv - mean[u];
+            mean[u] += d / (double)(b + 1);
+            m2[u] += d * (xv - mean[u]);
+          }
+          if (b + 1 < B) {
+            const double am = fabs((double)zm.im) * a.invG;
+            const double av = fabs((double)zv.im) * vfac;
+            const double xv = av / am - am * a.invn;
+            const double d = xv - mean[u];
+            mean[u] += d / (double)(b + 2);
+            m2[u] += d * (xv - mean[u]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int w = tid + u * nthr;
+    if (w < npts) {
+      const int t = w / n1, j1 = w - t * n1;
+      const double vv = sc * m2[u] / (double)(B - 1);
+      a.vmr[(int64_t)r * a.G + (q0 + t) * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+    }
+  }
+}
+
+template <typename T>
+void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+                       int64_t n_samples, int R, int r0, const T* gM, const T* gV,
+                       const double* cV, const double* vscale, const T* tw, double* vmr,
+                       hipStream_t s) {
+  VmrArgs<T> a;
+  const int n1 = g.n[0];
+  // 2048 points per workgroup where the grid allows it; PPT = points/thread
+  int TQ = 2048 / n1;
+  if (TQ < 1) TQ = 1;
+  if (TQ > g.Q) TQ = (int)g.Q;
+  const int npts = TQ * n1;
+  int ppt = (npts + kThreads - 1) / kThreads;
+  int NBP = 2048 / npts;  // bootstrap pairs per LDS batch
+  if (NBP < 1) NBP = 1;
+  if (NBP > (B + 1) / 2) NBP = (B + 1) / 2;
+  a.src = src;
+  a.src_rs = src_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.B = B;
+  a.invn = 1.0 / (double)n_samples;
+  a.invG = 1.0 / (double)g.G;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.ksum = g.ksum;
+  a.cV = cV;
+  a.vscale = vscale;
+  a.tw = (const cplx<T>*)tw;
+  a.vmr = vmr;
+  a.G = g.G;
+  a.Q = g.Q;
+  a.n1 = n1;
+  a.log2n1 = g.log2n[0];
+  a.H1 = g.H1;
+  a.TQ = TQ;
+  a.NBP = NBP;
+  a.ntiles = (int)(g.Q / TQ);
+  const unsigned blocks = (unsigned)((int64_t)R * a.ntiles);
+  const size_t lds = (size_t)2 * NBP * TQ * n1 * sizeof(cplx<T>);
+  if (ppt <= 1)
+    k_ladder_vmr<T, 1><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 2)
+    k_ladder_vmr<T, 2><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 4)
+    k_ladder_vmr<T, 4><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 8)
+    k_ladder_vmr<T, 8><<<blocks, kThreads, lds, s>>>(a);
+  else
+    k_ladder_vmr<T, 16><<<blocks, kThreads, lds, s>>>(a);
+}
+
+template void launch_ladder_vmr<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t, int,
+                                       int, const float*, const float*, const double*,
+                                       const double*, const float*, double*, hipStream_t);
+template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int64_t, int, int64_t,
+                                        int, int, const double*, const double*, const double*,
+                                        const double*, const double*, double*, hipStream_t);
+
+// -------------------------------------------------------------- convergence
+// find_stability (DirectSpace.jl:975-1031) applied rung after rung with the
+// history rotation of update_density_state_vmr! (GradeProEstimator.jl:617-625);
+// state stays in registers.  chosen = rung of the last density assignment, or
+// S if the point never converged (replace_nans!, GradeProEstimator.jl:1053-1061).
+__global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, const double* __restrict__ vmr,
+                                                          const double* __restrict__ dlogt,
+                                                          const double* __restrict__ params,
+                                                          int16_t* __restrict__ chosen,
+                                                          int16_t* __restrict__ sel) {
+  const double eps = params[0];
+  const int steps_low = (int)params[1];
+  const int steps_over = (int)params[2];
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < G;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double f1 = NAN, f2 = NAN, cur_min = NAN;
+    unsigned cl = 0, co = 0;
+    bool low = false;
+    int ch = S;
+    for (int r = 0; r < S; ++r) {
+      const double f = vmr[(int64_t)r * G + p];
+      const double dl = dlogt[r];
+      const double d1 = log(fabs(__ddiv_rn(__dsub_rn(f, f1), __dmul_rn(2.0, dl))));
+      const double d2 = log(fabs(__ddiv_rn(__dadd_rn(__dsub_rn(f, __dmul_rn(2.0, f1)), f2),
+                                           __dmul_rn(dl, dl))));
+      bool more_stable = false;
+      double new_min = NAN;
+      if (!low) {
+        if (d2 < eps) {
+          if (((d1 < cur_min) || isnan(cur_min)) && ((int)co <= steps_over)) {
+            more_stable = true;
+            new_min = d1;
+            co = 0;
+          } else {
+            co = (co + 1) & 0xFFFFu;
+          }
+        } else {
+          cl = (cl + 1) & 0xFFFFu;
+          if ((int)cl > steps_low) {
+            low = true;
+            cur_min = NAN;
+          }
+        }
+      } else {
+        if ((d2 < eps) && isnan(cur_min)) {
+          more_stable = true;
+          new_min = d1;
+        }
+      }
+      if (isnan(new_min)) new_min = cur_min;
+      cur_min = new_min;
+      if (more_stable) ch = r;
+      f2 = f1;
+      f1 = f;
+    }
+    chosen[p] = (int16_t)ch;
+    sel[p] = (int16_t)(ch == S ? S - 1 : ch);
+  }
+}
+
+void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
+                        const double* params, int16_t* chosen, int16_t* sel, hipStream_t s) {
+  k_convergence<<<grid_for(G, kThreads, 1 << 16), kThreads, 0, s>>>(G, S, vmr, dlogt, params, chosen,
+                                                                     sel);
+}
+
+// ------------------------------------------------------------ means select
+// Full-sample means (propagate_means!, ifft_means!, calculate_full_means!,
+// GradeProEstimator.jl:388-511) for two rungs per complex transform; each
+// point keeps the value of its selected rung.
+template <typename T>
+struct MeansArgs {
+  const cplx<T>* src;
+  int64_t src_rs;
+  int R, r0, S;
+  const T* gM;
+  int ksum;
+  const cplx<T>* tw;
+  const int16_t* sel;
+  double* density;
+  double invG;
+  int64_t Q;
+  int n1, log2n1, H1, TQ, ntiles;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_means_select(MeansArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* Z = (cplx<T>*)smem;
+  const int rp = blockIdx.x / a.ntiles;
+  const int tile = blockIdx.x - rp * a.ntiles;
+  const int ra = a.r0 + 2 * rp;
+  const int rb = ra + 1;
+  const bool has_b = (2 * rp + 1) < a.R;
+  const int64_t q0 = (int64_t)tile * a.TQ;
+  const int n1 = a.n1, H1 = a.H1, TQ = a.TQ;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const cplx<T>* sa = a.src + (int64_t)(2 * rp) * a.src_rs;
+  const cplx<T>* sb = a.src + (int64_t)(2 * rp + 1) * a.src_rs;
+  const T* ga = a.gM + (int64_t)ra * a.ksum;
+  const T* gb = a.gM + (int64_t)rb * a.ksum;
+  for (int id = tid; id < H1 * TQ; id += nthr) {
+    const int t = id % TQ, k1 = id / TQ;
+    const int64_t o = (int64_t)k1 * a.Q + q0 + t;
+    const cplx<T> A = cscale(sa[o], ga[k1]);
+    cplx<T> Bv = {(T)0, (T)0};
+    if (has_b) Bv = cscale(sb[o], gb[k1]);
+    put_pair(Z + (int64_t)t * n1, n1, k1, A, Bv);
+  }
+  __syncthreads();
+  lds_fft<1, T>(Z, a.log2n1, TQ, n1, a.tw + (n1 - 2), tid, nthr);
+  for (int w = tid; w < TQ * n1; w += nthr) {
+    const int t = w / n1, j1 = w - t * n1;
+    const int64_t p = (q0 + t) * n1 + j1;
+    const int sp = a.sel[p];
+    const cplx<T> z = Z[(int64_t)t * n1 + j1];
+    if (sp == ra)
+      a.density[p] = fabs((double)z.re) * a.invG;
+    else if (has_b && sp == rb)
+      a.density[p] = fabs((double)z.im) * a.invG;
+  }
+}
+
+template <typename T>
+void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int R,
+                         int r0, int S, const T* gM, const T* tw, const int16_t* sel,
+                         double* density, hipStream_t s) {
+  MeansArgs<T> a;
+  const int n1 = g.n[0];
+  int TQ = 2048 / n1;
+  if (TQ < 1) TQ = 1;
+  if (TQ > g.Q) TQ = (int)g.Q;
+  a.src = src;
+  a.src_rs = src_rung_stride;
+  a.R = R;
+  a.r0 = r0;
+  a.S = S;
+  a.gM = gM;
+  a.ksum = g.ksum;
+  a.tw = (const cplx<T>*)tw;
+  a.sel = sel;
+  a.density = density;
+  a.invG = 1.0 / (double)g.G;
+  a.Q = g.Q;
+  a.n1 = n1;
+  a.log2n1 = g.log2n[0];
+  a.H1 = g.H1;
+  a.TQ = TQ;
+  a.ntiles = (int)(g.Q / TQ);
+  const unsigned blocks = (unsigned)(((R + 1) / 2) * (int64_t)a.ntiles);
+  const size_t lds = (size_t)TQ * n1 * sizeof(cplx<T>);
+  k_means_select<T><<<blocks, kThreads, lds, s>>>(a);
+}
+
+template void launch_means_select<float>(const GridDev&, const cplx<float>*, int64_t, int, int, int,
+                                         const float*, const float*, const int16_t*, double*,
+                                         hipStream_t);
+template void launch_means_select<double>(const GridDev&, const cplx<double>*, int64_t, int, int, int,
+                                          const double*, const double*, const int16_t*, double*,
+                                          hipStream_t);
+
+// ---------------------------------------------------------------- Silverman
+// Two-pass sample moments with a fixed reduction tree (deterministic).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_moment_partial(int N, const double* __restrict__ x, int64_t n,
+                                                             const double* __restrict__ mean,
+                                                             double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  for (int i = 0; i < N; ++i) {
+    double acc = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+      const double v = x[k * N + i];
+      if (mean) {
+        const double d = v - mean[i];
+        acc += d * d;
+      } else {
+        acc += v;
+      }
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x * N + i] = s;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_moment_final(int N, int nblocks, int64_t n, double prefactor,
+                                                           const double* __restrict__ partial,
+                                                           double* mean_out, double* bw_out) {
+  __shared__ double red[kThreads];
+  for (int i = 0; i < N; ++i) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) acc += partial[k * N + i];
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+      if (bw_out)
+        bw_out[i] = prefactor * sqrt(s / (double)(n - 1));
+      else
+        mean_out[i] = s / (double)n;
+    }
+  }
+}
+
+void launch_silverman(int N, const double* x, int64_t n, double prefactor, double* partial,
+                      double* mean_out, double* bw_out, hipStream_t s) {
+  k_moment_partial<<<kMomentBlocks, kThreads, 0, s>>>(N, x, n, nullptr, partial);
+  k_moment_final<<<1, kThreads, 0, s>>>(N, kMomentBlocks, n, prefactor, partial, mean_out, nullptr);
+  k_moment_partial<<<kMomentBlocks, kThreads, 0, s>>>(N, x, n, mean_out, partial);
+  k_moment_final<<<1, kThreads, 0, s>>>(N, kMomentBlocks, n, prefactor, partial, nullptr, bw_out);
+}
+
+// ------------------------------------------------------------------- tables
+// Ladder scalars per rung (GradeProEstimator.jl:860-905, 954-966, 1074-1089;
+// DirectSpace.jl:564; FourierSpace.jl:224-225), with explicit rounding
+// intrinsics so the FP64 values follow the reference's operation order.
+__global__ void k_ladder_scalars(GridDev g, int S, int mode_steps, double steps_or_step,
+                                 const double* __restrict__ bw, int64_t n_samples, double alpha_s,
+                                 double alpha_os, double eps, double* times, double* dlogt,
+                                 double* vscale, double* cV, double* params) {
+  const int N = g.N;
+  double dt[kMaxDims], t0[kMaxDims];
+  for (int i = 0; i < N; ++i) {
+    dt[i] = mode_steps ? __ddiv_rn(bw[i], steps_or_step) : steps_or_step;
+    t0[i] = __ddiv_rn(g.h[i], 2.0);
+  }
+  const double n3 = (double)(n_samples * n_samples * n_samples);
+  for (int r = threadIdx.x; r < S; r += blockDim.x) {
+    double t[kMaxDims], tp[kMaxDims];
+    for (int i = 0; i < N; ++i) {
+      t[i] = __dmul_rn(dt[i], (double)r);
+      tp[i] = __dmul_rn(dt[i], (double)(r - 1));
+      times[r * N + i] = t[i];
+    }
+    double det_prev = 1.0, det_curr = 1.0, prodt = 1.0, det_t0 = 1.0;
+    for (int i = 0; i < N; ++i) {
+      const double t0s = __dmul_rn(t0[i], t0[i]);
+      const double p = (r == 0) ? t0s : __dadd_rn(t0s, __dmul_rn(tp[i], tp[i]));
+      const double c = __dadd_rn(t0s, __dmul_rn(t[i], t[i]));
+      const double q = __dadd_rn(__dmul_rn(t[i], t[i]), t0s);
+      det_prev = (i == 0) ? p : __dmul_rn(det_prev, p);
+      det_curr = (i == 0) ? c : __dmul_rn(det_curr, c);
+      prodt = (i == 0) ? q : __dmul_rn(prodt, q);
+      det_t0 = (i == 0) ? t0[i] : __dmul_rn(det_t0, t0[i]);
+    }
+    dlogt[r] = log(__ddiv_rn(det_curr, det_prev));
+    vscale[r] = __dmul_rn(pow(prodt, 1.5), n3);
+    cV[r] = __ddiv_rn(det_t0, sqrt(det_curr));
+  }
+  if (threadIdx.x == 0) {
+    // calculate_duration_steps on the last rung (ties-to-even rounding)
+    double sb = 0.0, so = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double tend = __dmul_rn(dt[i], (double)(S - 1));
+      const double ns = __ddiv_rn(tend, dt[i]);
+      const double b = __dmul_rn(alpha_s, ns), o = __dmul_rn(alpha_os, ns);
+      sb = (i == 0) ? b : __dadd_rn(sb, b);
+      so = (i == 0) ? o : __dadd_rn(so, o);
+    }
+    params[0] = eps;
+    params[1] = rint(__ddiv_rn(sb, (double)N));
+    params[2] = rint(__ddiv_rn(so, (double)N));
+  }
+}
+
+// Per-axis Gaussian factors: gM = exp(-(k t)^2 / 2), gV = sqrt(gM) so that
+// prod_i gM_i = exp(-sum_i (k_i t_i)^2 / 2) (propagate_time_cpu!).
+template <typename T>
+__global__ void k_gauss_tables(GridDev g, int S, const double* __restrict__ times,
+                               const double* __restrict__ kfreq, T* gM, T* gV) {
+  const int64_t total = (int64_t)S * g.ksum;
+  for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(id / g.ksum);
+    const int kk = (int)(id - (int64_t)r * g.ksum);
+    int ax = 0;
+    while (ax + 1 < g.N && kk >= g.koff[ax + 1]) ++ax;
+    const double a = __dmul_rn(kfreq[kk], times[r * g.N + ax]);
+    const double e = exp(__dmul_rn(-0.5, __dmul_rn(a, a)));
+    gM[id] = (T)e;
+    gV[id] = (T)sqrt(e);
+  }
+}
+
+template <typename T>
+void launch_tables(const GridDev& g, int S, int n_steps_mode, double n_steps_or_bwstep,
+                   const double* bw, const double* kfreq, int64_t n_samples, double alpha_s,
+                   double alpha_os, double eps, double* times, double* dlogt, double* vscale,
+                   double* cV, double* params, T* gM, T* gV, hipStream_t s) {
+  k_ladder_scalars<<<1, kThreads, 0, s>>>(g, S, n_steps_mode, n_steps_or_bwstep, bw, n_samples,
+                                          alpha_s, alpha_os, eps, times, dlogt, vscale, cV, params);
+  const int64_t total = (int64_t)S * g.ksum;
+  k_gauss_tables<T><<<grid_for(total), kThreads, 0, s>>>(g, S, times, kfreq, gM, gV);
+}
+
+template void launch_tables<float>(const GridDev&, int, int, double, const double*, const double*,
+                                   int64_t, double, double, double, double*, double*, double*,
+                                   double*, double*, float*, float*, hipStream_t);
+template void launch_tables<double>(const GridDev&, int, int, double, const double*, const double*,
+                                    int64_t, double, double, double, double*, double*, double*,
+                                    double*, double*, double*, double*, hipStream_t);
+
+}  // namespace pkde

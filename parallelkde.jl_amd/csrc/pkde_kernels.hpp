@@ -1,0 +1,88 @@
+// Kernel-level interface of the :gradepro HIP path (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pkde_fft.hpp"
+
+namespace pkde {
+
+constexpr int kMaxDims = 3;
+constexpr int kThreads = 256;
+
+// Grid geometry as the kernels see it (Grids.jl:67-87: lo, hi, spacing, n).
+struct GridDev {
+  int N;
+  int n[kMaxDims];
+  int log2n[kMaxDims];
+  int64_t G;   // prod n
+  int H1;      // n1/2 + 1 Hermitian half along axis 1
+  int64_t Q;   // n2 * n3 (pencils per axis-1 row set)
+  double lo[kMaxDims], hi[kMaxDims], h[kMaxDims];
+  int koff[kMaxDims];  // offset of axis i in per-rung Gaussian tables
+  int ksum;            // sum of n_i
+};
+
+// ---- launchers (implemented in pkde_kernels.hip) ----
+// Binning of B bootstrap resamples plus the full sample into acc[2B+1][G]
+// (field ids: [0,B) sums, [B,2B) squares, 2B full sample).  fixed != 0 uses
+// int64 fixed-point deposits, otherwise FP64 atomics.  err receives 1 if a
+// sample is outside the grid.
+void launch_binning(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
+                    uint64_t seed, int fixed, int frac_bits, void* acc, int* err,
+                    hipStream_t s);
+
+// acc -> forward transform along axis 1 in FP64, Hermitian halves written to
+// specD[fid][k1][q].
+void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double scaleM,
+                          double scaleV, int B, cplx<double>* specD, const double* tw64,
+                          hipStream_t s);
+
+// Generic batched pencil FFT along axis `axis` (>= 1, i.e. axis 2 or 3) of
+// [rung][field][k1][q] arrays.  Optional per-rung Gaussian factors.
+template <typename TI, typename TO, int DIR>
+void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
+                     int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
+                     int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
+                     hipStream_t s);
+
+template <typename TI, typename TO>
+void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream_t s);
+
+// Ladder: per rung r in [r0, r0+R) the bootstrap log-VMR per grid point
+// (propagate_bootstraps!, ifft_bootstraps!, calculate_vmr! fused).
+template <typename T>
+void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+                       int64_t n_samples, int R, int r0, const T* gM, const T* gV,
+                       const double* cV, const double* vscale, const T* tw, double* vmr,
+                       hipStream_t s);
+
+// Convergence state machine over all rungs -> chosen rung, selection rung.
+void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
+                        const double* params /* eps, steps_low, steps_over */,
+                        int16_t* chosen, int16_t* sel, hipStream_t s);
+
+// Full-sample means at each point's selected rung -> density.
+template <typename T>
+void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int R,
+                         int r0, int S, const T* gM, const T* tw, const int16_t* sel,
+                         double* density, hipStream_t s);
+
+// Silverman rule on device (DirectSpace.jl:1102-1111): bw[i] = prefactor * std_i.
+void launch_silverman(int N, const double* x, int64_t n, double prefactor, double* partial,
+                      double* mean_out, double* bw_out, hipStream_t s);
+constexpr int kMomentBlocks = 256;
+
+// Per-rung scalars (times, dlogt, VMR scaling, variance propagation factor,
+// steps_low / steps_over / eps) and the per-axis Gaussian tables.
+template <typename T>
+void launch_tables(const GridDev& g, int S, int n_steps_mode, double n_steps_or_bwstep,
+                   const double* bw, const double* kfreq, int64_t n_samples, double alpha_s,
+                   double alpha_os, double eps, double* times, double* dlogt, double* vscale,
+                   double* cV, double* params, T* gM, T* gV, hipStream_t s);
+
+void launch_fill_i32(int* p, int v, int64_t count, hipStream_t s);
+void launch_zero_bytes(void* p, int64_t bytes, hipStream_t s);
+
+}  // namespace pkde

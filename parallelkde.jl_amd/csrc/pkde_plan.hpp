@@ -1,0 +1,85 @@
+// Host-side plan of one :gradepro estimate on one device (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/parallelkde_hip.h"
+#include "pkde_kernels.hpp"
+
+struct pkde_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* tw64 = nullptr;  // exp(-2 pi i k / L) for L = 2..4096 at offset L-2
+  float* tw32 = nullptr;
+  std::mutex mu;
+  uint64_t chunk_bytes = 4ull << 30;  // per-plan budget of the rung-chunk buffer
+};
+
+namespace pkde {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+struct PlanBuffers {
+  double* x = nullptr;
+  int64_t x_cap = 0;
+  int32_t* idx = nullptr;
+  int64_t idx_cap = 0;
+  void* acc = nullptr;            // (2B+1) x G int64 / double
+  cplx<double>* specD = nullptr;  // (2B+1) x H1 x Q
+  void* spec = nullptr;           // FP32 copy (precision 0)
+  void* Y = nullptr;              // rung-chunk buffer
+  int64_t Y_elems = 0;
+  double* vmr = nullptr;  // S x G
+  int16_t* chosen = nullptr;
+  int16_t* sel = nullptr;
+  double* density = nullptr;
+  int* err = nullptr;
+  double* bw = nullptr;
+  double* mean = nullptr;
+  double* partial = nullptr;
+  double* kfreq = nullptr;
+  double* times = nullptr;
+  double* dlogt = nullptr;
+  double* vscale = nullptr;
+  double* cV = nullptr;
+  double* params = nullptr;
+  void* gM = nullptr;
+  void* gV = nullptr;
+  int S_alloc = 0;
+};
+
+}  // namespace pkde
+
+struct pkde_plan {
+  pkde_ctx* ctx = nullptr;
+  pkde::GridDev g{};
+  pkde_gradepro_params p{};
+  int B = 100;
+  int S = 251;            // rungs (n_steps + 1)
+  int steps_mode = 1;     // 1: n_steps, 0: bw_step
+  double steps_or_step = 250.0;
+  bool bw_given = false;
+  double bw_host[PKDE_MAX_DIMS] = {0, 0, 0};
+  double alpha_s = 0, alpha_os = 0.1, eps = 2.0;
+  int prec = 0;
+  int fixed = 1;
+  bool debug = false;
+  pkde::PlanBuffers d;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t last_n = 0;
+  int last_rc = PKDE_OK;
+};
+
+namespace pkde {
+// Enqueues one estimate of n device-resident samples; no host synchronisation.
+int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
+                 double* d_density, int16_t* d_chosen, hipStream_t s, bool timed);
+int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx);
+uint64_t plan_bytes(const pkde_plan* pl, int64_t n);
+}  // namespace pkde

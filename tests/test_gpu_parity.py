@@ -1,0 +1,294 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the FP64 oracle
+on the same samples and bootstrap draws.
+
+Tolerances (the floating-point bar of this path):
+  * binning: cell indices bit-exact (checked through deposits next to cell
+    edges); fixed-point deposit sums within 2^-40 of the largest deposit per
+    term; FP64-atomic sums within rtol 1e-12.
+  * convergence state machine on a given log-VMR table: chosen rung
+    bit-identical.
+  * whole estimate, FP64 mode: relative L1 density error <= 1e-9, chosen-rung
+    agreement 100 % on points whose density is >= 1e-3 of the maximum.
+  * whole estimate, FP32 spectra/FFT mode: relative L1 density error <= 1e-3
+    and chosen-rung agreement >= 99 % on those points (measured values are
+    printed; the tails below the FP32 noise floor decide on rounding noise in
+    the reference itself).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_common as W
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pk():
+    import parallelkde_rocm as pk
+    pk.context()
+    return pk
+
+
+@pytest.fixture(scope="module")
+def L(pk):
+    from parallelkde_rocm import _lib
+    return _lib.load()
+
+
+def _gpu_dirac(L, pk, g, x, idx, deterministic):
+    N = g.ndims
+    B, n = idx.shape
+    G = int(np.prod(O.grid_shape(g)))
+    sums = np.zeros(B * G)
+    sqs = np.zeros(B * G)
+    arr = lambda v, t: (t * N)(*v)
+    rc = L.pkde_dirac_sequences(pk.context().handle, N, arr([g.n[i] for i in range(N)], C.c_int64),
+                                arr([g.lo[i] for i in range(N)], C.c_double),
+                                arr([g.hi[i] for i in range(N)], C.c_double),
+                                arr([g.h[i] for i in range(N)], C.c_double),
+                                np.ascontiguousarray(x).ctypes.data, n,
+                                np.ascontiguousarray(idx, dtype=np.int32).ctypes.data, B,
+                                int(deterministic), sums.ctypes.data, sqs.ctypes.data)
+    return rc, sums.reshape(B, G), sqs.reshape(B, G)
+
+
+# ---------------------------------------------------------------- binning
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+@pytest.mark.parametrize("deterministic", [1, 0])
+def test_binning_reference_kat(L, pk, n_dims, deterministic):
+    say(f"[binning KAT] N={n_dims} det={deterministic}")
+    g = O.make_grid_step([-5.0] * n_dims, [5.0] * n_dims, [0.1] * n_dims, [101] * n_dims)
+    x = np.array([[0.05] * n_dims, [0.15] * n_dims])
+    idx = np.array([[0, 1], [1, 0]], dtype=np.int32)
+    rc, s, q = _gpu_dirac(L, pk, g, x, idx, deterministic)
+    assert rc == 0
+    _, es, eq = O.dirac_sequences(g, x, idx)
+    assert np.allclose(s, es, rtol=1e-11, atol=0)
+    assert np.allclose(q, eq, rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_binning_cell_edges_bit_exact(L, pk, n_dims):
+    """Samples at and one ulp around cell edges: any disagreement in
+    floor((x-lo)/h) or rem(x-lo, h) moves mass to another node."""
+    say(f"[binning edges] N={n_dims}")
+    rng = np.random.default_rng(20 + n_dims)
+    shape = [64, 32, 16][:n_dims]
+    lo = [-3.0, -1.7, -2.2][:n_dims]
+    hi = [3.0, 2.3, 0.9][:n_dims]
+    g = O.make_grid(lo, hi, shape)
+    n = 3000
+    cols = []
+    for i in range(n_dims):
+        k = rng.integers(1, shape[i] - 2, n)
+        base = lo[i] + k * g.h[i]
+        jitter = rng.choice([-1, 0, 1], n)
+        v = np.where(jitter < 0, np.nextafter(base, -np.inf), np.where(jitter > 0, np.nextafter(base, np.inf), base))
+        cols.append(v)
+    x = np.column_stack(cols)
+    idx = rng.integers(0, n, size=(3, n)).astype(np.int32)
+    _, es, eq = O.dirac_sequences(g, x, idx)
+    for det in (1, 0):
+        rc, s, q = _gpu_dirac(L, pk, g, x, idx, det)
+        assert rc == 0
+        tol = 2.0 ** -40 * es.max() * 4 if det else 1e-12 * es.max()
+        assert np.abs(s - es).max() <= tol
+        assert np.abs(q - eq).max() <= (2.0 ** -40 * eq.max() * 4 if det else 1e-12 * eq.max())
+        # identical support: no deposit moved to a neighbouring node
+        assert np.array_equal(s > 0, es > 0)
+
+
+def test_binning_out_of_grid(L, pk):
+    g = O.make_grid([-1.0], [1.0], [32])
+    x = np.array([[0.0], [1.0]])
+    rc, _, _ = _gpu_dirac(L, pk, g, x, np.array([[0, 1]], dtype=np.int32), 1)
+    assert rc == 2
+
+
+# ------------------------------------------------------------ convergence
+def test_convergence_kernel_matches_oracle(L, pk):
+    """The device state machine on a given log-VMR table equals the oracle's
+    find_stability applied rung by rung (DirectSpace.jl:975-1031)."""
+    say("[convergence] random tables")
+    rng = np.random.default_rng(5)
+    for trial, (G, S, eps, sl, so) in enumerate([(5000, 60, 2.0, 0, 25), (4000, 80, -0.5, 15, 25),
+                                                  (3000, 40, -0.5, 3, 5)]):
+        base = np.cumsum(rng.normal(0, 0.05, (S, G)), axis=0) + rng.normal(0, 1, G)
+        vmr = base + rng.normal(0, 0.02, (S, G))
+        vmr[rng.random((S, G)) < 0.01] = np.nan
+        dlogt = np.concatenate([[0.0], 2.0 * np.log(np.arange(2, S + 1) / np.arange(1, S))])
+        chosen = np.zeros(G, dtype=np.int16)
+        vmr_c = np.ascontiguousarray(vmr)
+        rc = L.pkde_gradepro_convergence(pk.context().handle, G, S, vmr_c.ctypes.data,
+                                         dlogt.ctypes.data, eps, sl, so, chosen.ctypes.data)
+        assert rc == 0
+        # oracle, rung by rung
+        dens = np.full(G, np.nan)
+        means = np.ones(G)
+        f1, f2 = np.full(G, np.nan), np.full(G, np.nan)
+        mins = np.full(G, np.nan)
+        cl, co, lf = np.zeros(G, np.uint16), np.zeros(G, np.uint16), np.zeros(G, np.uint8)
+        ch = np.full(G, S, dtype=np.int16)
+        for r in range(S):
+            O.lib().og_identify_convergence(G, dens, means, np.ascontiguousarray(vmr[r]), f1, f2, dlogt[r],
+                                            eps, sl, so, mins, cl, co, lf, ch.ctypes.data, r, 1)
+        assert np.array_equal(chosen, ch), (trial, np.mean(chosen == ch))
+
+
+# ----------------------------------------------------------- whole estimate
+def _run_pair(pk, grid_spec, x, B, steps, precision, seed=3, bw_final=None):
+    lo, hi, n = grid_spec
+    og = O.make_grid(lo, hi, n)
+    idx = O.bootstrap_indices(seed, x.shape[0], B)
+    grid = pk.initialize_grid([(lo[i], hi[i], n[i]) for i in range(len(n))])
+    est = pk.initialize_estimation(x, grid=grid)
+    kw = dict(n_bootstraps=B, n_steps=steps, precision=precision, bootstrap_idx=idx)
+    if bw_final is not None:
+        kw["bw_final"] = bw_final
+    pk.estimate_density(est, "gradepro", **kw)
+    ref = O.gradepro(og, x, idx_bs=idx, n_bootstraps=B, n_steps=steps, threads=8, tables=True,
+                     bw_final=bw_final)
+    assert ref.rc == 0
+    res = est.last_result
+    m = W.compare(res.density, res.chosen_rung, ref.density, ref.chosen, steps + 1)
+    return est, res, ref, m
+
+
+CASES = [
+    ("1d-512", ([-5.0], [5.0], [512]), lambda: W.gaussian_1d(10000, 1), 100, 250),
+    ("2d-64", ([-6.0, -6.0], [6.0, 6.0], [64, 64]), lambda: W.mixture_2d(20000, 2), 40, 120),
+    ("3d-32", ([-4.0] * 3, [4.0] * 3, [32, 32, 32]), lambda: W.correlated_3d(20000, 3), 20, 60),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_estimate_matches_oracle(pk, case, precision):
+    name, spec, gen, B, steps = case
+    say(f"[estimate] {name} {precision} B={B} S={steps + 1}")
+    x = gen()
+    est, res, ref, m = _run_pair(pk, spec, x, B, steps, precision)
+    say(f"  {name} {precision}: rel L1 {m['rel_l1']:.3e} rel Linf {m['rel_linf']:.3e} "
+        f"chosen agree {m['agree_all']:.5f} (sig {m['agree_sig']:.5f} of {m['n_sig']}) "
+        f"gpu {res.ms_total:.2f} ms")
+    # ladder defaults (App. A1): steps_low/steps_over/eps
+    assert (res.steps_low, res.steps_over) == (ref.info.steps_low, ref.info.steps_over)
+    assert res.eps == ref.info.eps
+    assert np.allclose(res.bw_final, [ref.info.bw_final[i] for i in range(len(spec[2]))], rtol=1e-12)
+    assert np.all(np.isfinite(res.density))
+    if precision == "fp64":
+        assert m["rel_l1"] <= 1e-9
+        assert m["agree_sig"] == 1.0
+    else:
+        assert m["rel_l1"] <= 1e-3
+        assert m["agree_sig"] >= 0.99
+
+
+def test_vmr_table_matches_oracle_fp64(pk):
+    """Per-rung log-VMR (calculate_scaled_vmr!) against the oracle's, on
+    points where the full-sample means are significant."""
+    say("[vmr table] 1d fp64")
+    x = W.gaussian_1d(5000, 4)
+    est, res, ref, m = _run_pair(pk, ([-5.0], [5.0], [256]), x, 30, 80, "fp64")
+    plan = next(iter(est._plans.values()))
+    vmr = plan.debug_vmr(res.n_rungs)
+    sig = ref.means >= 1e-4 * ref.means.max()
+    sig[0] = False  # rung 0 (t = 0) is rounding-noise-dominated wherever a cell is empty
+    d = np.abs(vmr - ref.vmr)[sig & np.isfinite(ref.vmr)]
+    say(f"  max |dvmr| = {d.max():.3e}")
+    assert d.max() < 1e-7
+
+
+def test_estimate_out_of_grid_raises(pk):
+    x = np.array([[0.0], [0.5], [5.0]])
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-5.0, 5.0, 64)]))
+    with pytest.raises(pk.ArgumentError):
+        pk.estimate_density(est, "gradepro", n_bootstraps=4, n_steps=10)
+
+
+def test_non_power_of_two_grid_rejected(pk):
+    x = np.random.default_rng(0).standard_normal((100, 1))
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-6.0, 6.0, 300)]))
+    with pytest.raises(pk.ArgumentError):
+        pk.estimate_density(est, "gradepro")
+
+
+def test_deterministic_repeat(pk):
+    """Fixed-point binning + fixed reduction orders: bitwise repeatable."""
+    x = W.mixture_2d(30000, 9)
+    grid = pk.initialize_grid([(-6.0, 6.0, 128), (-6.0, 6.0, 128)])
+    e1 = pk.initialize_estimation(x, grid=grid)
+    pk.estimate_density(e1, "gradepro", n_bootstraps=50, n_steps=100, seed=42)
+    d1 = pk.get_density(e1).copy()
+    c1 = e1.last_result.chosen_rung.copy()
+    pk.estimate_density(e1, "gradepro", n_bootstraps=50, n_steps=100, seed=42)
+    assert np.array_equal(d1, pk.get_density(e1))
+    assert np.array_equal(c1, e1.last_result.chosen_rung)
+
+
+def test_end_to_end_mise_1d(pk):
+    """gradepro_tests.jl:485-511 on a power-of-two grid: 10,000 N(0,1)
+    samples, 256 points over [-6, 6], MISE < 5e-5."""
+    from scipy.stats import norm
+    x = W.gaussian_1d(10000, 17, lo=-6.0, hi=6.0)
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-6.0, 6.0, 256)]))
+    pk.estimate_density(est, "gradepro")
+    xs = np.linspace(-6, 6, 256)
+    mise = ((pk.get_density(est) - norm.pdf(xs)) ** 2).sum() * (xs[1] - xs[0]) / 256
+    say(f"  MISE = {mise:.3e}")
+    assert mise < 5e-5
+
+
+def test_batch_equals_plain(pk):
+    rng = np.random.default_rng(1)
+    items = []
+    for k in range(4):
+        d = 1 + k % 2
+        x = W.gaussian_1d(3000 + 500 * k, 30 + k) if d == 1 else W.mixture_2d(4000, 40 + k)
+        g = pk.initialize_grid([(-5.0, 5.0, 256)] if d == 1 else [(-6.0, 6.0, 64)] * 2)
+        items.append((x, g, dict(n_bootstraps=20, n_steps=50, seed=100 + k)))
+    outs = pk.estimate_batch(items)
+    for (x, g, kw), dens in zip(items, outs):
+        est = pk.initialize_estimation(x, grid=g)
+        pk.estimate_density(est, "gradepro", **kw)
+        assert np.array_equal(dens, pk.get_density(est))
+
+
+def test_server_alone_equals_in_mix(pk):
+    """Continuous batching: each result is identical alone and in the mix."""
+    reqs = []
+    for k in range(6):
+        d = 1 + k % 3
+        if d == 1:
+            x, g = W.gaussian_1d(4000, 60 + k), pk.initialize_grid([(-5.0, 5.0, 512)])
+        elif d == 2:
+            x, g = W.mixture_2d(8000, 60 + k), pk.initialize_grid([(-6.0, 6.0, 64)] * 2)
+        else:
+            x, g = W.correlated_3d(8000, 60 + k), pk.initialize_grid([(-4.0, 4.0, 32)] * 3)
+        reqs.append((x, g, dict(n_bootstraps=16, n_steps=40, seed=7 + k)))
+    srv = pk.Server(max_inflight=4)
+    alone = []
+    for x, g, kw in reqs:
+        t = srv.submit(x, g, **kw)
+        alone.append(srv.wait(t))
+    tickets = [srv.submit(x, g, **kw) for x, g, kw in reqs]
+    mixed = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    say(f"  server stats {st}")
+    for (da, ca), (dm, cm) in zip(alone, mixed):
+        assert np.array_equal(da, dm) and np.array_equal(ca, cm)
+    # and equal to the plain path
+    x, g, kw = reqs[0]
+    est = pk.initialize_estimation(x, grid=g)
+    pk.estimate_density(est, "gradepro", **kw)
+    assert np.array_equal(alone[0][0], pk.get_density(est))

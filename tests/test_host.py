@@ -1,0 +1,126 @@
+"""CPU-side tests of the boundary: the C-ABI library loads and exports every
+entry point its header declares, and the Python mirror of the reference API
+behaves like ParallelKDE.jl's host logic (grid construction, validation,
+error mapping).  No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "parallelkde_hip.h")
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(pkde_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header_symbols():
+    from parallelkde_rocm import _lib
+    L = _lib.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(_lib.EXPORTED_SYMBOLS)
+    assert L.pkde_version() == 1
+
+
+def test_params_default_matches_reference_defaults():
+    from parallelkde_rocm import _lib
+    p = _lib.Params()
+    _lib.load().pkde_gradepro_params_default(C.byref(p))
+    assert p.n_bootstraps == 100 and p.n_steps == 250  # GradeProEstimator.jl:705, :902
+    assert np.isnan(p.bw_step) and all(np.isnan(p.bw_final[i]) for i in range(3))
+    assert np.isnan(p.alpha_s) and np.isnan(p.alpha_os) and np.isnan(p.eps)
+    assert p.deterministic_binning == 1 and p.precision == 0
+
+
+def test_no_device_fails_loudly():
+    import parallelkde_rocm as pk
+    from parallelkde_rocm import _lib
+    h = C.c_void_p()
+    rc = _lib.load().pkde_context_create(0, C.byref(h))
+    if rc == 0:
+        _lib.load().pkde_context_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == _lib.NO_DEVICE
+    est = pk.initialize_estimation(np.random.default_rng(0).standard_normal((50, 1)),
+                                   grid=pk.initialize_grid([(-5.0, 5.0, 64)]))
+    with pytest.raises(pk.DeviceUnavailableError):
+        pk.estimate_density(est, "gradepro")
+
+
+def test_grid_geometry_kat():
+    """grid_tests.jl:1-36: sizes, spacings, bounds, fftgrid = 2pi fftfreq."""
+    import parallelkde_rocm as pk
+    for n_dims in (1, 2, 3):
+        g = pk.initialize_grid([(-1.0, 1.0, 21)] * n_dims)
+        assert g.shape == (21,) * n_dims and g.ndims == n_dims
+        assert np.allclose(pk.spacings(g), 0.1, rtol=1e-15)
+        assert np.array_equal(pk.bounds(g), np.array([[-1.0] * n_dims, [1.0] * n_dims]))
+        assert np.array_equal(pk.low_bounds(g), [-1.0] * n_dims)
+        assert np.array_equal(pk.high_bounds(g), [1.0] * n_dims)
+        assert np.allclose(pk.initial_bandwidth(g), 0.05)
+        for k in pk.fftgrid(g):
+            assert np.allclose(k, 2 * np.pi * np.fft.fftfreq(21, 0.1), rtol=1e-14, atol=1e-14)
+        coords = g.get_coordinates()
+        assert coords.shape == (n_dims,) + (21,) * n_dims
+
+
+def test_find_grid_padding():
+    """grid_tests.jl:29-33: data at +-1 gives bounds +-1.2 (10 % padding)."""
+    import parallelkde_rocm as pk
+    for n_dims in (1, 2, 3):
+        data = np.array([[-1.0] * n_dims, [1.0] * n_dims])
+        g = pk.find_grid(data)
+        assert np.allclose(pk.bounds(g), np.array([[-1.2] * n_dims, [1.2] * n_dims]), rtol=1e-15)
+        assert g.shape == (pk.FIND_GRID_DEFAULT_POINTS,) * n_dims
+
+
+def test_initialize_estimation_contract():
+    """ParallelKDE.jl:102-158 and KDEs.jl:62-111."""
+    import parallelkde_rocm as pk
+    x = np.random.default_rng(1).standard_normal((100, 2))
+    est = pk.initialize_estimation(x, grid=True, grid_ranges=[(-5.0, 5.0, 64), (-4.0, 4.0, 32)])
+    assert pk.has_grid(est) and pk.get_grid(est).shape == (64, 32)
+    assert est.kde.density.shape == (64, 32) and np.all(np.isnan(est.kde.density))
+    assert est.kde.get_nsamples() == 100
+    with pytest.raises(pk.ArgumentError):
+        pk.initialize_estimation(x, grid=False)  # no grid and no dims
+    est2 = pk.initialize_estimation(x, grid=False, dims=(8, 8))
+    assert not pk.has_grid(est2)
+    with pytest.raises(pk.ArgumentError):
+        pk.initialize_estimation(x, grid=True, device=":cuda")  # not a device of this package
+    with pytest.raises(pk.ArgumentError):
+        pk.initialize_kde(x, (10, 10, 10))  # dimension mismatch
+
+
+def test_estimate_argument_validation():
+    import parallelkde_rocm as pk
+    x = np.random.default_rng(2).standard_normal((100, 1))
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-5.0, 5.0, 64)]))
+    with pytest.raises(KeyError):
+        pk.estimate_density(est, "nosuchestimator")
+    with pytest.raises(pk.ArgumentError):
+        pk.estimate_density(est, "gradepro", method="serial")  # ensure_valid_implementation
+    with pytest.raises(TypeError):
+        pk.estimate_density(est, "gradepro", bandwidth=0.3)
+
+
+def test_get_density_normalisation():
+    """ParallelKDE.jl:189-202: normalise in place by sum * prod(h)."""
+    import parallelkde_rocm as pk
+    x = np.random.default_rng(3).standard_normal((10, 1))
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-5.0, 5.0, 11)]))
+    est.kde.density[:] = 2.0
+    d = pk.get_density(est, normalize=True)
+    assert np.isclose(d.sum() * 1.0, 1.0)
+    est3 = pk.initialize_estimation(x, grid=False, dims=(11,))
+    with pytest.raises(pk.ArgumentError):
+        pk.get_density(est3, normalize=True)
+    est3.kde.density[:] = 1.0
+    assert np.isclose(pk.get_density(est3, normalize=True, dx=0.5).sum(), 2.0)
