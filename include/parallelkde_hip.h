@@ -65,8 +65,9 @@ typedef struct {
   double eps;                          /* NaN: 2.0 (1-D) / -0.5 (>=2-D) (:541-550) */
   uint64_t rng_seed;                   /* Philox key for in-kernel bootstrap draws */
   int32_t deterministic_binning;       /* 1: fixed-point integer deposits (default) */
-  int32_t precision;                   /* 0: FP32 spectra + FFT, FP64 statistics
-                                          1: FP64 throughout */
+  int32_t precision;                   /* 1: FP64 throughout (default)
+                                          0: FP32 spectra + ladder FFTs, FP64
+                                             binning/forward FFT/statistics */
 } pkde_gradepro_params;
 
 /* Timing / accounting of one estimate (optional output). */

@@ -262,7 +262,7 @@ void pkde_gradepro_params_default(pkde_gradepro_params* p) {
   p->eps = NAN;
   p->rng_seed = 0x5eed;
   p->deterministic_binning = 1;
-  p->precision = 0;
+  p->precision = 1;
 }
 
 int pkde_context_create(int device, pkde_ctx** out) {

@@ -342,7 +342,7 @@ class GradeProResult:
 
 
 def _params(ndims, n_bootstraps=100, bw_final=None, bw_step=None, n_steps=None, alpha_s=None,
-            alpha_os=None, eps=None, seed=0x5EED, precision="fp32", deterministic=True):
+            alpha_os=None, eps=None, seed=0x5EED, precision="fp64", deterministic=True):
     p = _lib.Params()
     _lib.load().pkde_gradepro_params_default(_lib.C.byref(p))
     if int(n_bootstraps) < 0:

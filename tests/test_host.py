@@ -36,7 +36,7 @@ def test_params_default_matches_reference_defaults():
     assert p.n_bootstraps == 100 and p.n_steps == 250  # GradeProEstimator.jl:705, :902
     assert np.isnan(p.bw_step) and all(np.isnan(p.bw_final[i]) for i in range(3))
     assert np.isnan(p.alpha_s) and np.isnan(p.alpha_os) and np.isnan(p.eps)
-    assert p.deterministic_binning == 1 and p.precision == 0
+    assert p.deterministic_binning == 1 and p.precision == 1
 
 
 def test_no_device_fails_loudly():
