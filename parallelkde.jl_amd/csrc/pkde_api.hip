@@ -67,6 +67,19 @@ static int frac_bits_for(int64_t n) {
 
 static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; }
 
+// The rung-chunk buffer has y_units() equal parts (two in 3-D: the axis-2
+// result and the transposed axis-3 result); each part holds at least one
+// rung of the 2B bootstrap fields.
+static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
+
+static uint64_t y_part_bytes(const pkde_plan* pl) {
+  const size_t rs = real_size(pl->prec);
+  const uint64_t per_rung = (uint64_t)2 * pl->B * fsize(pl) * 2 * rs;
+  uint64_t part = std::min<uint64_t>((uint64_t)pl->S * per_rung,
+                                     std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes / y_units(pl)));
+  return std::max<uint64_t>(part, (uint64_t)fsize(pl) * 2 * rs * 2);
+}
+
 uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   const int64_t G = pl->g.G, F = 2 * (int64_t)pl->B + 1, fs = fsize(pl);
   const size_t rs = real_size(pl->prec);
@@ -78,12 +91,7 @@ uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   b += (uint64_t)pl->S * G * 8;                   // log-VMR table
   b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
   b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
-  if (pl->g.N >= 2) {
-    const uint64_t per_rung = (uint64_t)2 * pl->B * fs * 2 * rs;
-    uint64_t Y = std::min<uint64_t>((uint64_t)pl->S * per_rung, std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes));
-    Y = std::max<uint64_t>(Y, (uint64_t)fs * 2 * rs);
-    b += Y;
-  }
+  if (pl->g.N >= 2) b += (uint64_t)y_units(pl) * y_part_bytes(pl);
   return b;
 }
 
@@ -110,9 +118,7 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   if ((rc = dalloc((void**)&d.specD, (size_t)F * fs * sizeof(cplx<double>), "spectra"))) return rc;
   if (!pl->prec && (rc = dalloc(&d.spec, (size_t)F * fs * sizeof(cplx<float>), "spectra fp32"))) return rc;
   if (pl->g.N >= 2 && !d.Y) {
-    const uint64_t per_rung = (uint64_t)2 * pl->B * fs * 2 * rs;
-    uint64_t Y = std::min<uint64_t>((uint64_t)pl->S * per_rung, std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes));
-    Y = std::max<uint64_t>(Y, (uint64_t)fs * 2 * rs);
+    const uint64_t Y = (uint64_t)y_units(pl) * y_part_bytes(pl);
     if ((rc = dalloc(&d.Y, Y, "rung chunk buffer"))) return rc;
     d.Y_elems = (int64_t)(Y / (2 * rs));
   }
@@ -181,7 +187,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   }
   for (int ax = 1; ax < N; ++ax)
     launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
-                                        nullptr, nullptr, 0, pl->ctx->tw64, s);
+                                        nullptr, nullptr, 0, pl->ctx->tw64, 0, s);
   const cplx<T>* spec;
   if (sizeof(T) == 4) {
     launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
@@ -197,15 +203,30 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   if (N == 1) {
     launch_ladder_vmr<T>(g, spec, 0, B, n, S, 0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
   } else {
+    // axis 2 (and 3) inverse passes write the transposed layout [q][k1] the
+    // row kernel reads; in 3-D the axis-2 result goes through a second buffer
     const int64_t per_rung = 2 * (int64_t)B * fs;
-    int Rc = (int)std::max<int64_t>(1, std::min<int64_t>(S, d.Y_elems / per_rung));
+    const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512;
+    const int64_t cap = d.Y_elems / y_units(pl);
+    if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
+    const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
     cplx<T>* Y = (cplx<T>*)d.Y;
+    cplx<T>* Y2 = Y + cap;
     for (int r0 = 0; r0 < S; r0 += Rc) {
       const int R = std::min(Rc, S - r0);
-      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw, s);
-      if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw, s);
-      launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+      if (N == 2) {
+        launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
+                                 rows ? 1 : 0, s);
+      } else {
+        launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y2, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
+                                 0, s);
+        launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B,
+                                 tw, rows ? 1 : 0, s);
+      }
+      if (rows)
+        launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+      else
+        launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
     }
   }
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
@@ -216,15 +237,16 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   if (N == 1) {
     launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, d.sel, d_density, s);
   } else {
-    int Rm = (int)std::max<int64_t>(1, std::min<int64_t>(S, d.Y_elems / fs));
+    if (d.Y_elems < 2 * fs) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than two rungs");
+    int Rm = (int)std::min<int64_t>(S, d.Y_elems / fs);
     if (Rm > 1 && (Rm & 1)) --Rm;
     cplx<T>* Y = (cplx<T>*)d.Y;
     const int fM = 2 * B;
     for (int r0 = 0; r0 < S; r0 += Rm) {
       const int R = std::min(Rm, S - r0);
-      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, 0, s);
       if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, 0, s);
       launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
     }
   }

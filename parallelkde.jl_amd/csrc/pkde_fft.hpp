@@ -201,3 +201,77 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 }
 
 }  // namespace pkde
+
+namespace pkde {
+
+// ------------------------------------------------------ wave-local passes
+// When a pencil is owned by lanes of a single wavefront, LDS exchanges
+// between passes need only wavefront-scope ordering: LDS operations of one
+// wave execute in order, so a fence that stops the compiler from moving
+// LDS accesses across the pass boundary is sufficient.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham pass of radix R over the lane's own pencil `pb` (length L),
+// the lane being thread t of the pencil's L/E threads.  All lanes of the
+// wave call it (active = false for lanes without a pencil).
+template <int R, int E, int DIR, typename T>
+__device__ __forceinline__ void wave_pass(cplx<T>* pb, int L, int Ns, int t, bool active,
+                                          const cplx<T>* __restrict__ tw) {
+  const int tpp = L / E;
+  const int nbfly = L / R;
+  const int twstep = L / (Ns * R);
+  cplx<T> v[E];
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = t + u * tpp;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u * R + r] = pb[j + r * nbfly];
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = t + u * tpp;
+      const int jm = j % Ns;
+      cplx<T>* w = v + u * R;
+      if (jm != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> f = tw[jm * r * twstep];
+          if (DIR > 0) f.im = -f.im;
+          w[r] = cmul(w[r], f);
+        }
+      }
+      dft_reg<R, DIR, T>(w);
+      const int out0 = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pb[out0 + r * Ns] = w[r];
+    }
+  }
+  wave_sync();
+}
+
+// Wave-local FFT of the lane's pencil (L = 2^log2L, 16 <= L <= 1024, so the
+// L/16 threads of a pencil sit in one wavefront).  Natural order in and out.
+template <int DIR, typename T>
+__device__ __forceinline__ void wave_fft(cplx<T>* pb, int log2L, int t, bool active,
+                                         const cplx<T>* __restrict__ tw) {
+  const int L = 1 << log2L;
+  int Ns = 1, rem = log2L;
+  while (rem >= 4) {
+    wave_pass<16, 16, DIR, T>(pb, L, Ns, t, active, tw);
+    Ns *= 16;
+    rem -= 4;
+  }
+  if (rem == 3) wave_pass<8, 16, DIR, T>(pb, L, Ns, t, active, tw);
+  else if (rem == 2) wave_pass<4, 16, DIR, T>(pb, L, Ns, t, active, tw);
+  else if (rem == 1) wave_pass<2, 16, DIR, T>(pb, L, Ns, t, active, tw);
+}
+
+}  // namespace pkde

@@ -23,6 +23,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace pkde {
 
 // ------------------------------------------------------------------ utilities
@@ -259,9 +261,13 @@ void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double s
 }
 
 // ------------------------------------------------------------- axis passes
-// Pencils along axis a (a = 1 or 2, i.e. grid axis 2 or 3) of fields laid out
-// [k1][q] with q = j2 + n2*j3.  Axis 2 pencils are contiguous; axis 3 pencils
-// have stride n2 and consecutive pencils are adjacent in memory.
+// Pencils along grid axis 2 or 3 of fields stored [k1][q], q = j2 + n2*j3.
+// One field is viewed as [k1][o][e][i]: e runs along the pencil (stride I),
+// o and i are the remaining outer / inner indices.  A workgroup owns TK
+// consecutive k1 x TI consecutive i for one (rung, field, o); each pencil
+// lives in one wavefront (L/16 lanes) so the FFT passes need no block barrier.
+// The result is written either in the same layout or transposed to [q][k1],
+// which gives the axis-1 stage contiguous rows.
 template <typename T>
 struct AxisArgs {
   const cplx<T>* src;
@@ -274,61 +280,91 @@ struct AxisArgs {
   const T* gV;
   int fsplit, ksum, goff;
   int L, log2L;
-  int64_t es, NI, OS, PF;
-  int PW, tiles_per_field;
+  int64_t I, O, Q;
+  int H1, TK, TI, ktiles, itiles;
+  int transposed;
   const cplx<T>* tw;
 };
 
-template <typename T, int DIR>
+template <typename T, int DIR, bool WAVE>
 __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = (cplx<T>*)smem;
-  const int per_rung = a.nf * a.tiles_per_field;
-  const int rr = blockIdx.x / per_rung;
-  const int rem = blockIdx.x - rr * per_rung;
-  const int fl = rem / a.tiles_per_field;
-  const int tile = rem - fl * a.tiles_per_field;
+  int64_t bid = blockIdx.x;
+  const int it = (int)(bid % a.itiles);
+  bid /= a.itiles;
+  const int kt = (int)(bid % a.ktiles);
+  bid /= a.ktiles;
+  const int64_t o = bid % a.O;
+  bid /= a.O;
+  const int fl = (int)(bid % a.nf);
+  const int rr = (int)(bid / a.nf);
   const int f = a.fid0 + fl;
-  const int64_t p0 = (int64_t)tile * a.PW;
-  const int npen = (int)((a.PF - p0) < a.PW ? (a.PF - p0) : a.PW);
+  const int k0 = kt * a.TK;
+  const int TKe = (a.H1 - k0) < a.TK ? (a.H1 - k0) : a.TK;
+  const int64_t i0 = (int64_t)it * a.TI;
+  const int TI = a.TI, L = a.L;
+  const int npen = TKe * TI;
   const T* gt = nullptr;
   if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
   const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize;
   cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.fsize;
+  const int64_t obase = o * L * a.I;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int L = a.L;
   const int total = npen * L;
+  // load: lanes run along the contiguous index (e if I == 1, else i)
   for (int id = tid; id < total; id += nthr) {
-    int pw, e;
-    if (a.NI == 1) {
-      pw = id / L;
-      e = id - pw * L;
+    int kk, ii, e;
+    if (TI == 1) {
+      kk = id / L;
+      e = id - kk * L;
+      ii = 0;
     } else {
-      e = id / npen;
-      pw = id - e * npen;
+      ii = id % TI;
+      e = (id / TI) % L;
+      kk = id / (TI * L);
     }
-    const int64_t p = p0 + pw;
-    const int64_t po = p / a.NI, pin = p - po * a.NI;
-    const int64_t addr = po * a.OS + pin + (int64_t)e * a.es;
-    cplx<T> v = src[addr];
+    const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+    cplx<T> v = src[(int64_t)(k0 + kk) * a.Q + q];
     if (gt) v = cscale(v, gt[e]);
-    buf[pw * L + e] = v;
+    buf[(kk * TI + ii) * L + e] = v;
   }
   __syncthreads();
-  lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
-  for (int id = tid; id < total; id += nthr) {
-    int pw, e;
-    if (a.NI == 1) {
-      pw = id / L;
-      e = id - pw * L;
-    } else {
-      e = id / npen;
-      pw = id - e * npen;
+  if (WAVE) {
+    const int lp = L >> 4;  // lanes per pencil
+    const int lane = tid & 63, wv = tid >> 6;
+    const int ppw = 64 / lp;
+    const int nw = nthr >> 6;
+    for (int pb = wv * ppw; pb < npen; pb += nw * ppw) {
+      const int p = pb + lane / lp;
+      const bool act = p < npen;
+      wave_fft<DIR, T>(buf + (act ? p : 0) * L, a.log2L, lane % lp, act, a.tw + (L - 2));
     }
-    const int64_t p = p0 + pw;
-    const int64_t po = p / a.NI, pin = p - po * a.NI;
-    const int64_t addr = po * a.OS + pin + (int64_t)e * a.es;
-    dst[addr] = buf[pw * L + e];
+    __syncthreads();
+  } else {
+    lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
+  }
+  for (int id = tid; id < total; id += nthr) {
+    int kk, ii, e;
+    if (a.transposed) {
+      kk = id % TKe;
+      ii = (id / TKe) % TI;
+      e = id / (TKe * TI);
+    } else if (TI == 1) {
+      kk = id / L;
+      e = id - kk * L;
+      ii = 0;
+    } else {
+      ii = id % TI;
+      e = (id / TI) % L;
+      kk = id / (TI * L);
+    }
+    const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+    const cplx<T> v = buf[(kk * TI + ii) * L + e];
+    if (a.transposed)
+      dst[q * a.H1 + k0 + kk] = v;
+    else
+      dst[(int64_t)(k0 + kk) * a.Q + q] = v;
   }
 }
 
@@ -336,7 +372,7 @@ template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     hipStream_t s) {
+                     int transposed, hipStream_t s) {
   static_assert(sizeof(TI) == sizeof(TO), "axis passes keep the element type");
   AxisArgs<TI> a;
   a.src = src;
@@ -357,32 +393,39 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.goff = g.koff[axis];
   a.L = g.n[axis];
   a.log2L = g.log2n[axis];
-  const int64_t n2 = g.n[1];
-  if (axis == 1) {
-    a.es = 1;
-    a.NI = 1;
-    a.OS = n2;
-  } else {
-    a.es = n2;
-    a.NI = n2;
-    a.OS = g.Q;
+  a.Q = g.Q;
+  a.H1 = g.H1;
+  if (axis == 1) {  // grid axis 2: contiguous, outer index j3
+    a.I = 1;
+    a.O = g.n[2];
+  } else {          // grid axis 3: stride n2, inner index j2
+    a.I = g.n[1];
+    a.O = 1;
   }
-  a.PF = a.fsize / a.L;
-  int PW = 4096 / a.L;
-  if (PW < 1) PW = 1;
-  if (PW > a.PF) PW = (int)a.PF;
-  a.PW = PW;
-  a.tiles_per_field = (int)((a.PF + PW - 1) / PW);
+  const int per_block = 4096 / a.L > 0 ? 4096 / a.L : 1;  // pencils per workgroup
+  if (a.I == 1) {
+    a.TI = 1;
+  } else {
+    a.TI = (int)std::min<int64_t>(a.I, std::max(1, std::min(8, per_block)));
+  }
+  a.TK = std::max(1, std::min(g.H1, per_block / a.TI));
+  a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+  a.itiles = (int)(a.I / a.TI);
+  a.transposed = transposed;
   a.tw = (const cplx<TI>*)tw;
-  const int64_t blocks = (int64_t)R * nf * a.tiles_per_field;
-  const size_t lds = (size_t)PW * a.L * sizeof(cplx<TI>);
-  k_axis_fft<TI, DIR><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
+  const size_t lds = (size_t)a.TK * a.TI * a.L * sizeof(cplx<TI>);
+  if (a.L >= 16 && a.L <= 1024)
+    k_axis_fft<TI, DIR, true><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  else
+    k_axis_fft<TI, DIR, false><<<(unsigned)blocks, kThreads, lds, s>>>(a);
 }
 
 #define PKDE_AXIS_INST(TI, TO, DIR)                                                            \
   template void launch_axis_fft<TI, TO, DIR>(const GridDev&, int, const cplx<TI>*, int64_t, int,  \
                                              cplx<TO>*, int64_t, int, int, int, int, int,        \
-                                             const TI*, const TI*, int, const TI*, hipStream_t);
+                                             const TI*, const TI*, int, const TI*, int,          \
+                                             hipStream_t);
 PKDE_AXIS_INST(double, double, -1)
 PKDE_AXIS_INST(double, double, 1)
 PKDE_AXIS_INST(float, float, 1)
@@ -587,6 +630,141 @@ template void launch_ladder_vmr<float>(const GridDev&, const cplx<float>*, int64
 template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int64_t, int, int64_t,
                                         int, int, const double*, const double*, const double*,
                                         const double*, const double*, double*, hipStream_t);
+
+// ------------------------------------------------- ladder VMR, row per wave
+// Same computation as k_ladder_vmr for grids with Q >= 64 rows and
+// 16 <= n1 <= 512, reading the axis-2/3-transformed fields in the transposed
+// layout [rung][field][q][k1] so every row is contiguous.  Each wavefront owns
+// W = 512/n1 rows of one rung, builds the M and V pencils of two bootstraps
+// per row in its own LDS slice, transforms them with wave-local passes and
+// accumulates the FP64 Welford statistics of its 8 points per lane; the
+// wavefronts of a workgroup never synchronise with each other.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int PPT = 8;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int n1 = a.n1, H1 = a.H1, B = a.B;
+  const int W = 512 / n1;            // rows per wave
+  const int lp = n1 >> 4;            // lanes per pencil
+  cplx<T>* Z = (cplx<T>*)smem + (int64_t)wv * 2 * W * n1;
+  const int64_t rows_per_rung = a.Q / W;
+  const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  if (gw >= (int64_t)a.R * rows_per_rung) return;
+  const int rr = (int)(gw / rows_per_rung);
+  const int64_t q0 = (gw - (int64_t)rr * rows_per_rung) * W;
+  const int r = a.r0 + rr;
+  const T* gm1 = a.gM + (int64_t)r * a.ksum;
+  const T* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  const int my_pen = lane / lp;      // pencil of this lane: ty * W + row
+  const int my_t = lane - my_pen * lp;
+
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+  const int nload = 2 * W * H1;
+  for (int b = 0; b < B; b += 2) {
+    const bool has2 = b + 1 < B;
+    for (int id = lane; id < nload; id += 64) {
+      const int k1 = id % H1;
+      const int rest = id / H1;
+      const int w = rest % W, ty = rest / W;
+      const int fa = ty * B + b;
+      const T gk = ty ? gv1[k1] : gm1[k1];
+      const int64_t o = (q0 + w) * H1 + k1;
+      const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + o], gk);
+      cplx<T> Bv = {(T)0, (T)0};
+      if (has2) Bv = cscale(src[(int64_t)(fa + 1) * a.fsize + o], gk);
+      put_pair(Z + (ty * W + w) * n1, n1, k1, A, Bv);
+    }
+    wave_sync();
+    wave_fft<1, T>(Z + my_pen * n1, a.log2n1, my_t, true, a.tw + (n1 - 2));
+    const double inv1 = 1.0 / (double)(b + 1), inv2 = 1.0 / (double)(b + 2);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int pt = lane + 64 * u;
+      const int w = pt / n1, j1 = pt - w * n1;
+      const cplx<T> zm = Z[w * n1 + j1];
+      const cplx<T> zv = Z[(W + w) * n1 + j1];
+      {
+        const double am = fabs((double)zm.re) * a.invG;
+        const double av = fabs((double)zv.re) * vfac;
+        const double xv = av / am - am * a.invn;
+        const double d = xv - mean[u];
+        mean[u] += d * inv1;
+        m2[u] += d * (xv - mean[u]);
+      }
+      if (has2) {
+        const double am = fabs((double)zm.im) * a.invG;
+        const double av = fabs((double)zv.im) * vfac;
+        const double xv = av / am - am * a.invn;
+        const double d = xv - mean[u];
+        mean[u] += d * inv2;
+        m2[u] += d * (xv - mean[u]);
+      }
+    }
+    wave_sync();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int pt = lane + 64 * u;
+    const int w = pt / n1, j1 = pt - w * n1;
+    const double vv = sc * m2[u] / (double)(B - 1);
+    a.vmr[(int64_t)r * a.G + (q0 + w) * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+  }
+}
+
+template <typename T>
+void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+                        int64_t n_samples, int R, int r0, const T* gM, const T* gV,
+                        const double* cV, const double* vscale, const T* tw, double* vmr,
+                        hipStream_t s) {
+  VmrArgs<T> a;
+  const int n1 = g.n[0];
+  a.src = src;
+  a.src_rs = src_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.B = B;
+  a.invn = 1.0 / (double)n_samples;
+  a.invG = 1.0 / (double)g.G;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.ksum = g.ksum;
+  a.cV = cV;
+  a.vscale = vscale;
+  a.tw = (const cplx<T>*)tw;
+  a.vmr = vmr;
+  a.G = g.G;
+  a.Q = g.Q;
+  a.n1 = n1;
+  a.log2n1 = g.log2n[0];
+  a.H1 = g.H1;
+  a.TQ = 512 / n1;
+  a.NBP = 1;
+  a.ntiles = 0;
+  const int64_t waves = (int64_t)R * (g.Q / a.TQ);
+  const int wpb = kThreads / 64;
+  const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
+  const size_t lds = (size_t)wpb * 2 * 512 * sizeof(cplx<T>);
+  k_ladder_rows<T><<<blocks, kThreads, lds, s>>>(a);
+}
+
+template void launch_ladder_rows<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t,
+                                        int, int, const float*, const float*, const double*,
+                                        const double*, const float*, double*, hipStream_t);
+template void launch_ladder_rows<double>(const GridDev&, const cplx<double>*, int64_t, int,
+                                         int64_t, int, int, const double*, const double*,
+                                         const double*, const double*, const double*, double*,
+                                         hipStream_t);
 
 // -------------------------------------------------------------- convergence
 // find_stability (DirectSpace.jl:975-1031) applied rung after rung with the

@@ -45,7 +45,7 @@ template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     hipStream_t s);
+                     int transposed, hipStream_t s);
 
 template <typename TI, typename TO>
 void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream_t s);
@@ -57,6 +57,14 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
                        int64_t n_samples, int R, int r0, const T* gM, const T* gV,
                        const double* cV, const double* vscale, const T* tw, double* vmr,
                        hipStream_t s);
+
+// Row-per-wave form of the same ladder step for Q >= 64 rows and
+// 16 <= n1 <= 512, reading the transposed layout [rung][field][q][k1].
+template <typename T>
+void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+                        int64_t n_samples, int R, int r0, const T* gM, const T* gV,
+                        const double* cV, const double* vscale, const T* tw, double* vmr,
+                        hipStream_t s);
 
 // Convergence state machine over all rungs -> chosen rung, selection rung.
 void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
