@@ -257,21 +257,24 @@ __device__ __forceinline__ void wave_pass(cplx<T>* pb, int L, int Ns, int t, boo
   wave_sync();
 }
 
-// Wave-local FFT of the lane's pencil (L = 2^log2L, 16 <= L <= 1024, so the
-// L/16 threads of a pencil sit in one wavefront).  Natural order in and out.
-template <int DIR, typename T>
+// Wave-local FFT of the lane's pencil: L = 2^log2L with E <= L <= 64 * E,
+// so the L/E threads of a pencil sit in one wavefront.  E = 16 gives radix-16
+// passes; E = 8 halves the registers per lane (radix-8 passes), which the FP64
+// kernels need for occupancy.  Natural order in and out.
+template <int DIR, typename T, int E = 16>
 __device__ __forceinline__ void wave_fft(cplx<T>* pb, int log2L, int t, bool active,
                                          const cplx<T>* __restrict__ tw) {
   const int L = 1 << log2L;
+  constexpr int LE = E == 16 ? 4 : 3;
   int Ns = 1, rem = log2L;
-  while (rem >= 4) {
-    wave_pass<16, 16, DIR, T>(pb, L, Ns, t, active, tw);
-    Ns *= 16;
-    rem -= 4;
+  while (rem >= LE) {
+    wave_pass<E, E, DIR, T>(pb, L, Ns, t, active, tw);
+    Ns *= E;
+    rem -= LE;
   }
-  if (rem == 3) wave_pass<8, 16, DIR, T>(pb, L, Ns, t, active, tw);
-  else if (rem == 2) wave_pass<4, 16, DIR, T>(pb, L, Ns, t, active, tw);
-  else if (rem == 1) wave_pass<2, 16, DIR, T>(pb, L, Ns, t, active, tw);
+  if (rem == 3) wave_pass<8, E, DIR, T>(pb, L, Ns, t, active, tw);
+  else if (rem == 2) wave_pass<4, E, DIR, T>(pb, L, Ns, t, active, tw);
+  else if (rem == 1) wave_pass<2, E, DIR, T>(pb, L, Ns, t, active, tw);
 }
 
 }  // namespace pkde

@@ -288,6 +288,7 @@ struct AxisArgs {
 
 template <typename T, int DIR, bool WAVE>
 __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
+  constexpr int E = sizeof(T) == 8 ? 8 : 16;  // elements per lane in the FFT
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = (cplx<T>*)smem;
   int64_t bid = blockIdx.x;
@@ -331,14 +332,14 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
   }
   __syncthreads();
   if (WAVE) {
-    const int lp = L >> 4;  // lanes per pencil
+    const int lp = L / E;  // lanes per pencil
     const int lane = tid & 63, wv = tid >> 6;
     const int ppw = 64 / lp;
     const int nw = nthr >> 6;
     for (int pb = wv * ppw; pb < npen; pb += nw * ppw) {
       const int p = pb + lane / lp;
       const bool act = p < npen;
-      wave_fft<DIR, T>(buf + (act ? p : 0) * L, a.log2L, lane % lp, act, a.tw + (L - 2));
+      wave_fft<DIR, T, E>(buf + (act ? p : 0) * L, a.log2L, lane % lp, act, a.tw + (L - 2));
     }
     __syncthreads();
   } else {
@@ -402,7 +403,9 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
     a.I = g.n[1];
     a.O = 1;
   }
-  const int per_block = 4096 / a.L > 0 ? 4096 / a.L : 1;  // pencils per workgroup
+  // pencils per workgroup: one per L/E lanes (E = 8 in FP64, 16 in FP32)
+  const int Ef = sizeof(TI) == 8 ? 8 : 16;
+  const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
   if (a.I == 1) {
     a.TI = 1;
   } else {
@@ -415,7 +418,7 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.tw = (const cplx<TI>*)tw;
   const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
   const size_t lds = (size_t)a.TK * a.TI * a.L * sizeof(cplx<TI>);
-  if (a.L >= 16 && a.L <= 1024)
+  if (a.L >= Ef && a.L <= 64 * Ef)
     k_axis_fft<TI, DIR, true><<<(unsigned)blocks, kThreads, lds, s>>>(a);
   else
     k_axis_fft<TI, DIR, false><<<(unsigned)blocks, kThreads, lds, s>>>(a);
@@ -639,15 +642,15 @@ template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int
 // per row in its own LDS slice, transforms them with wave-local passes and
 // accumulates the FP64 Welford statistics of its 8 points per lane; the
 // wavefronts of a workgroup never synchronise with each other.
-template <typename T>
+template <typename T, int E, int PPT>
 __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int PPT = 8;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int n1 = a.n1, H1 = a.H1, B = a.B;
-  const int W = 512 / n1;            // rows per wave
-  const int lp = n1 >> 4;            // lanes per pencil
+  const int W = 64 * PPT / n1;       // rows per wave (PPT points per lane)
+  const int lp = n1 / E;             // lanes per pencil
+  const int ppc = 64 / lp;           // pencils per wave-wide FFT call
   cplx<T>* Z = (cplx<T>*)smem + (int64_t)wv * 2 * W * n1;
   const int64_t rows_per_rung = a.Q / W;
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
@@ -659,8 +662,6 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   const T* gv1 = a.gV + (int64_t)r * a.ksum;
   const cplx<T>* src = a.src + (int64_t)rr * a.src_rs;
   const double vfac = a.cV[r] * a.invG;
-  const int my_pen = lane / lp;      // pencil of this lane: ty * W + row
-  const int my_t = lane - my_pen * lp;
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -684,7 +685,11 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       put_pair(Z + (ty * W + w) * n1, n1, k1, A, Bv);
     }
     wave_sync();
-    wave_fft<1, T>(Z + my_pen * n1, a.log2n1, my_t, true, a.tw + (n1 - 2));
+    for (int c = 0; c < 2 * W; c += ppc) {
+      const int p = c + lane / lp;
+      const bool act = p < 2 * W;
+      wave_fft<1, T, E>(Z + (act ? p : 0) * n1, a.log2n1, lane % lp, act, a.tw + (n1 - 2));
+    }
     const double inv1 = 1.0 / (double)(b + 1), inv2 = 1.0 / (double)(b + 2);
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -721,6 +726,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   }
 }
 
+// Points per lane: 4 in FP64 (8 KB of LDS and ~half the registers of 8
+// points per wave), 8 in FP32; a 512-point row needs 8.
 template <typename T>
 void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
@@ -728,6 +735,9 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
                         hipStream_t s) {
   VmrArgs<T> a;
   const int n1 = g.n[0];
+  constexpr int E = sizeof(T) == 8 ? 8 : 16;
+  const int ppt = (sizeof(T) == 8 && n1 <= 256) ? 4 : 8;
+  const int W = 64 * ppt / n1;
   a.src = src;
   a.src_rs = src_rung_stride;
   a.fsize = (int64_t)g.H1 * g.Q;
@@ -748,14 +758,17 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   a.n1 = n1;
   a.log2n1 = g.log2n[0];
   a.H1 = g.H1;
-  a.TQ = 512 / n1;
+  a.TQ = W;
   a.NBP = 1;
   a.ntiles = 0;
-  const int64_t waves = (int64_t)R * (g.Q / a.TQ);
+  const int64_t waves = (int64_t)R * (g.Q / W);
   const int wpb = kThreads / 64;
   const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
-  const size_t lds = (size_t)wpb * 2 * 512 * sizeof(cplx<T>);
-  k_ladder_rows<T><<<blocks, kThreads, lds, s>>>(a);
+  const size_t lds = (size_t)wpb * 2 * W * n1 * sizeof(cplx<T>);
+  if (ppt == 4)
+    k_ladder_rows<T, E, 4><<<blocks, kThreads, lds, s>>>(a);
+  else
+    k_ladder_rows<T, E, 8><<<blocks, kThreads, lds, s>>>(a);
 }
 
 template void launch_ladder_rows<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t,
