@@ -74,7 +74,7 @@ static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
 
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
-  const uint64_t per_rung = (uint64_t)2 * pl->B * fsize(pl) * 2 * rs;
+  const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.Q * pl->g.H1p * 2 * rs;
   uint64_t part = std::min<uint64_t>((uint64_t)pl->S * per_rung,
                                      std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes / y_units(pl)));
   return std::max<uint64_t>(part, (uint64_t)fsize(pl) * 2 * rs * 2);
@@ -205,7 +205,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   } else {
     // axis 2 (and 3) inverse passes write the transposed layout [q][k1] the
     // row kernel reads; in 3-D the axis-2 result goes through a second buffer
-    const int64_t per_rung = 2 * (int64_t)B * fs;
+    const int64_t per_rung = 2 * (int64_t)B * g.Q * g.H1p;  // padded rows
     const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512;
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
@@ -396,6 +396,7 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     g.G *= g.n[i];
   }
   g.H1 = g.n[0] / 2 + 1;
+  g.H1p = (g.H1 + 7) / 8 * 8;
   g.Q = g.G / g.n[0];
   pl->B = p.n_bootstraps;
   pl->prec = p.precision ? 1 : 0;

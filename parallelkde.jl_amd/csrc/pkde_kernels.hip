@@ -274,6 +274,7 @@ struct AxisArgs {
   cplx<T>* dst;
   int64_t src_rs, dst_rs;  // rung strides (elements)
   int64_t fsize;           // H1 * Q
+  int64_t dst_fsize;       // H1 * Q, or Q * H1p when transposed
   int fid0, nf, src_fid0, dst_fid0;
   int R, r0;
   const T* gM;
@@ -281,7 +282,7 @@ struct AxisArgs {
   int fsplit, ksum, goff;
   int L, log2L;
   int64_t I, O, Q;
-  int H1, TK, TI, ktiles, itiles;
+  int H1, H1p, TK, TI, ktiles, itiles;
   int transposed;
   const cplx<T>* tw;
 };
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
   const T* gt = nullptr;
   if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
   const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize;
-  cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.fsize;
+  cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
   const int64_t obase = o * L * a.I;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int total = npen * L;
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
     const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
     const cplx<T> v = buf[(kk * TI + ii) * L + e];
     if (a.transposed)
-      dst[q * a.H1 + k0 + kk] = v;
+      dst[q * a.H1p + k0 + kk] = v;
     else
       dst[(int64_t)(k0 + kk) * a.Q + q] = v;
   }
@@ -396,6 +397,8 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.log2L = g.log2n[axis];
   a.Q = g.Q;
   a.H1 = g.H1;
+  a.H1p = g.H1p;
+  a.dst_fsize = transposed ? (int64_t)g.Q * g.H1p : a.fsize;
   if (axis == 1) {  // grid axis 2: contiguous, outer index j3
     a.I = 1;
     a.O = g.n[2];
@@ -483,6 +486,7 @@ struct VmrArgs {
   double* vmr;
   int64_t G, Q;
   int n1, log2n1, H1, TQ, NBP, ntiles;
+  int64_t rowp;  // row stride of the transposed layout (rows kernel)
 };
 
 template <typename T, int PPT>
@@ -678,7 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       const int w = rest % W, ty = rest / W;
       const int fa = ty * B + b;
       const T gk = ty ? gv1[k1] : gm1[k1];
-      const int64_t o = (q0 + w) * H1 + k1;
+      const int64_t o = (q0 + w) * a.rowp + k1;
       const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + o], gk);
       cplx<T> Bv = {(T)0, (T)0};
       if (has2) Bv = cscale(src[(int64_t)(fa + 1) * a.fsize + o], gk);
@@ -740,7 +744,8 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   const int W = 64 * ppt / n1;
   a.src = src;
   a.src_rs = src_rung_stride;
-  a.fsize = (int64_t)g.H1 * g.Q;
+  a.fsize = (int64_t)g.Q * g.H1p;
+  a.rowp = g.H1p;
   a.B = B;
   a.invn = 1.0 / (double)n_samples;
   a.invG = 1.0 / (double)g.G;

@@ -18,6 +18,7 @@ struct GridDev {
   int log2n[kMaxDims];
   int64_t G;   // prod n
   int H1;      // n1/2 + 1 Hermitian half along axis 1
+  int H1p;     // H1 rounded up to 8: row stride of the transposed [q][k1] layout
   int64_t Q;   // n2 * n3 (pencils per axis-1 row set)
   double lo[kMaxDims], hi[kMaxDims], h[kMaxDims];
   int koff[kMaxDims];  // offset of axis i in per-rung Gaussian tables
