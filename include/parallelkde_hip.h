@@ -150,8 +150,10 @@ int pkde_dirac_sequences(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
                          int32_t B, int deterministic, double *sums_out, double *squares_out);
 
 /* ---- batched form: many independent estimates in one submission ----
- * Every request's kernels go to one stream in sequence without host
- * synchronisation; results are written when the call returns. */
+ * Requests are spread over up to PKDE_BATCH_STREAMS (default 8) device
+ * streams and run concurrently; device buffers stay resident between calls.
+ * Results are written when the call returns.  No counterpart in the
+ * reference, which estimates one dataset per call. */
 typedef struct {
   int32_t ndims;
   int64_t n_per_axis[PKDE_MAX_DIMS];

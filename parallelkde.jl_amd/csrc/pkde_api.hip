@@ -333,6 +333,13 @@ int pkde_context_create(int device, pkde_ctx** out) {
 void pkde_context_destroy(pkde_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  for (auto& ln : ctx->lanes) {
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+    for (auto& kv : ln.plans) pkde_gradepro_plan_destroy(kv.second);
+    if (ln.stream) (void)hipStreamDestroy(ln.stream);
+  }
+  ctx->lanes.clear();
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->tw64) (void)hipFree(ctx->tw64);
   if (ctx->tw32) (void)hipFree(ctx->tw32);
@@ -544,6 +551,7 @@ int pkde_gradepro_estimate_device(pkde_plan* pl, const double* d_samples, int64_
 
 int pkde_gradepro_check(pkde_plan* pl) {
   if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  if (!pl->d.err || pl->last_n == 0) return fail(PKDE_INVALID_ARGUMENT, "no estimate has run");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
   int err = 0;

@@ -462,6 +462,18 @@ __device__ __forceinline__ void put_pair(cplx<T>* row, int n1, int k1, cplx<T> A
   if (k1 > 0 && 2 * k1 < n1) row[n1 - k1] = {A.re + Bv.im, Bv.re - A.im};
 }
 
+// r = |s|/|m| - |m|/n, the per-bootstrap variance-to-mean ratio
+// (DirectSpace.jl:550-552 rearranged: (|s|/n - (|m|/n)^2) / (|m|/n)).  The
+// quotient uses the hardware reciprocal refined by two Newton steps (within
+// 1 ulp) instead of the IEEE division sequence; |m| = 0 gives NaN as the
+// division does once it reaches the Welford update.
+__device__ __forceinline__ double vmr_ratio(double av, double am, double invn) {
+  double inv = __builtin_amdgcn_rcp(am);
+  inv = fma(fma(-am, inv, 1.0), inv, inv);
+  inv = fma(fma(-am, inv, 1.0), inv, inv);
+  return fma(av, inv, -am * invn);
+}
+
 // --------------------------------------------------------------- ladder VMR
 // One workgroup owns TQ consecutive axis-1 rows (TQ*n1 grid points) of one
 // rung and walks all bootstraps in order, two at a time per complex
@@ -548,7 +560,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
           {
             const double am = fabs((double)zm.re) * a.invG;
             const double av = fabs((double)zv.re) * vfac;
-            const double xv = av / am - am * a.invn;
+            const double xv = vmr_ratio(av, am, a.invn);
             const double d = xv - mean[u];
             mean[u] += d / (double)(b + 1);
             m2[u] += d * (xv - mean[u]);
@@ -556,7 +568,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
           if (b + 1 < B) {
             const double am = fabs((double)zm.im) * a.invG;
             const double av = fabs((double)zv.im) * vfac;
-            const double xv = av / am - am * a.invn;
+            const double xv = vmr_ratio(av, am, a.invn);
             const double d = xv - mean[u];
             mean[u] += d / (double)(b + 2);
             m2[u] += d * (xv - mean[u]);
@@ -704,7 +716,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       {
         const double am = fabs((double)zm.re) * a.invG;
         const double av = fabs((double)zv.re) * vfac;
-        const double xv = av / am - am * a.invn;
+        const double xv = vmr_ratio(av, am, a.invn);
         const double d = xv - mean[u];
         mean[u] += d * inv1;
         m2[u] += d * (xv - mean[u]);
@@ -712,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       if (has2) {
         const double am = fabs((double)zm.im) * a.invG;
         const double av = fabs((double)zv.im) * vfac;
-        const double xv = av / am - am * a.invn;
+        const double xv = vmr_ratio(av, am, a.invn);
         const double d = xv - mean[u];
         mean[u] += d * inv2;
         m2[u] += d * (xv - mean[u]);

@@ -10,6 +10,16 @@
 #include "../../include/parallelkde_hip.h"
 #include "pkde_kernels.hpp"
 
+struct pkde_plan;
+
+// Stream lane of the batched form: its own stream and the plans (device
+// buffers) it keeps resident between calls, keyed by grid and parameters.
+struct pkde_batch_lane {
+  hipStream_t stream = nullptr;
+  std::vector<std::pair<std::vector<unsigned char>, pkde_plan*>> plans;
+  double work = 0.0;
+};
+
 struct pkde_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -17,6 +27,10 @@ struct pkde_ctx {
   float* tw32 = nullptr;
   std::mutex mu;
   uint64_t chunk_bytes = 4ull << 30;  // per-plan budget of the rung-chunk buffer
+  // batched form
+  std::vector<pkde_batch_lane> lanes;
+  unsigned char* stage = nullptr;  // pinned host staging
+  size_t stage_bytes = 0;
 };
 
 namespace pkde {

@@ -1,8 +1,10 @@
 // Batched and serving forms of the :gradepro estimator.
 //
-// Batched: a list of independent requests is enqueued back to back on one
-// stream with no host synchronisation in between; plans (and their device
-// buffers) are reused across requests of the same shape.
+// Batched: a list of independent requests is spread over stream lanes
+// (longest-processing-time first) and enqueued with no host synchronisation
+// until the end, so small estimates run concurrently.  Each lane keeps its
+// plans (device buffers) resident between calls; host data moves through a
+// pinned staging buffer so asynchronous copies never block the host.
 //
 // Serving: a worker thread runs continuous batching over `max_inflight`
 // device streams.  A request is admitted as soon as a stream slot is free and
@@ -61,55 +63,132 @@ double now_ms() {
 
 }  // namespace
 
+// Estimated device work of a request: the ladder (grid points x rungs x
+// bootstraps) plus binning (draws x corners).
+static double request_work(const pkde_request& r) {
+  double G = 1.0;
+  for (int i = 0; i < r.ndims; ++i) G *= (double)r.n_per_axis[i];
+  const double B = r.params.n_bootstraps > 0 ? r.params.n_bootstraps : 100;
+  const double S = r.params.n_steps > 0 ? r.params.n_steps + 1 : 251;
+  return G * S * B + (double)r.n * B * (double)(1 << r.ndims);
+}
+
+static std::vector<unsigned char> key_bytes(const pkde_request& r) {
+  PlanKey k = key_of(r);
+  const unsigned char* p = (const unsigned char*)&k;
+  return std::vector<unsigned char>(p, p + sizeof(k));
+}
+
 extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const pkde_request* reqs,
                                             int32_t* status_out) {
   if (!ctx || count < 0 || (count > 0 && !reqs)) return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
-  hipStream_t s = ctx->stream;
-  std::vector<std::pair<PlanKey, pkde_plan*>> plans;
-  std::vector<pkde_plan*> used(count, nullptr);
-  std::vector<int*> errs(count, nullptr);
-  std::vector<int> errv(count, 0);
-  int first_rc = PKDE_OK;
+  // stream lanes: requests are spread over independent streams so that small
+  // estimates run concurrently; each lane keeps its plans resident
+  int nl = 8;
+  if (const char* e = getenv("PKDE_BATCH_STREAMS")) nl = std::max(1, std::min(16, atoi(e)));
+  while ((int)ctx->lanes.size() < nl) {
+    pkde_batch_lane ln;
+    rc = check_hip(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking), "stream");
+    if (rc) return rc;
+    ctx->lanes.push_back(ln);
+  }
+  // pinned staging: samples in, density / chosen / flag out
+  std::vector<size_t> off_x(count), off_d(count), off_c(count), off_e(count);
+  size_t total = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = total;
+    total += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
   for (int i = 0; i < count; ++i) {
     const pkde_request& r = reqs[i];
-    int st = PKDE_OK;
-    if (!r.samples || !r.density_out || r.n < 2) st = fail(PKDE_INVALID_ARGUMENT, "invalid request");
+    int64_t G = 1;
+    for (int a = 0; a < r.ndims && a < PKDE_MAX_DIMS; ++a) G *= r.n_per_axis[a];
+    off_x[i] = take((size_t)std::max<int64_t>(r.n, 0) * std::max(r.ndims, 0) * 8);
+    off_d[i] = take((size_t)std::max<int64_t>(G, 0) * 8);
+    off_c[i] = take((size_t)std::max<int64_t>(G, 0) * 2);
+    off_e[i] = take(4);
+  }
+  if (total > ctx->stage_bytes) {
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
+    rc = check_hip(hipHostMalloc((void**)&ctx->stage, total, hipHostMallocDefault), "pinned staging");
+    if (rc) return rc;
+    ctx->stage_bytes = total;
+  }
+  // longest-processing-time assignment of requests to lanes
+  std::vector<int> order(count), lane_of(count, 0);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(),
+            [&](int x, int y) { return request_work(reqs[x]) > request_work(reqs[y]); });
+  for (auto& ln : ctx->lanes) ln.work = 0.0;
+  for (int i : order) {
+    int best = 0;
+    for (int l = 1; l < nl; ++l)
+      if (ctx->lanes[l].work < ctx->lanes[best].work) best = l;
+    lane_of[i] = best;
+    ctx->lanes[best].work += request_work(reqs[i]);
+  }
+  std::vector<int> st(count, PKDE_OK);
+  std::vector<pkde_plan*> used(count, nullptr);
+  int first_rc = PKDE_OK;
+  for (int i : order) {
+    const pkde_request& r = reqs[i];
+    pkde_batch_lane& ln = ctx->lanes[lane_of[i]];
+    hipStream_t s = ln.stream;
+    if (!r.samples || !r.density_out || r.n < 2 || r.ndims < 1 || r.ndims > PKDE_MAX_DIMS) {
+      st[i] = fail(PKDE_INVALID_ARGUMENT, "invalid request");
+      continue;
+    }
     pkde_plan* pl = nullptr;
-    if (st == PKDE_OK) {
-      PlanKey k = key_of(r);
-      for (auto& kv : plans)
-        if (kv.first == k) pl = kv.second;
-      if (!pl) {
-        st = make_plan(ctx, r, &pl);
-        if (st == PKDE_OK) plans.push_back({k, pl});
+    std::vector<unsigned char> k = key_bytes(r);
+    for (auto& kv : ln.plans)
+      if (kv.first == k) pl = kv.second;
+    if (!pl) {
+      st[i] = make_plan(ctx, r, &pl);
+      if (st[i]) continue;
+      ln.plans.push_back({k, pl});
+    }
+    used[i] = pl;
+    memcpy(ctx->stage + off_x[i], r.samples, (size_t)r.n * r.ndims * 8);
+    st[i] = plan_reserve(pl, r.n, false);
+    if (!st[i])
+      st[i] = check_hip(hipMemcpyAsync(pl->d.x, ctx->stage + off_x[i], (size_t)r.n * r.ndims * 8,
+                                       hipMemcpyHostToDevice, s), "upload");
+    if (!st[i]) st[i] = plan_enqueue(pl, pl->d.x, r.n, nullptr, pl->d.density, pl->d.chosen, s, false);
+    if (!st[i])
+      st[i] = check_hip(hipMemcpyAsync(ctx->stage + off_d[i], pl->d.density, (size_t)pl->g.G * 8,
+                                       hipMemcpyDeviceToHost, s), "download");
+    if (!st[i])
+      st[i] = check_hip(hipMemcpyAsync(ctx->stage + off_c[i], pl->d.chosen, (size_t)pl->g.G * 2,
+                                       hipMemcpyDeviceToHost, s), "download");
+    if (!st[i])
+      st[i] = check_hip(hipMemcpyAsync(ctx->stage + off_e[i], pl->d.err, 4, hipMemcpyDeviceToHost, s),
+                        "flag");
+  }
+  for (int l = 0; l < nl; ++l) {
+    int e = check_hip(hipStreamSynchronize(ctx->lanes[l].stream), "batch");
+    if (e && !rc) rc = e;
+  }
+  for (int i = 0; i < count; ++i) {
+    if (st[i] == PKDE_OK && used[i]) {
+      const int64_t G = used[i]->g.G;
+      int err = 0;
+      memcpy(&err, ctx->stage + off_e[i], 4);
+      if (err & 1)
+        st[i] = fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid.");
+      else {
+        memcpy(reqs[i].density_out, ctx->stage + off_d[i], (size_t)G * 8);
+        if (reqs[i].chosen_out) memcpy(reqs[i].chosen_out, ctx->stage + off_c[i], (size_t)G * 2);
       }
     }
-    if (st == PKDE_OK) st = plan_reserve(pl, r.n, false);
-    if (st == PKDE_OK)
-      st = check_hip(hipMemcpyAsync(pl->d.x, r.samples, (size_t)r.n * r.ndims * 8, hipMemcpyHostToDevice, s),
-                     "upload");
-    if (st == PKDE_OK) st = plan_enqueue(pl, pl->d.x, r.n, nullptr, pl->d.density, pl->d.chosen, s, false);
-    if (st == PKDE_OK)
-      st = check_hip(hipMemcpyAsync(r.density_out, pl->d.density, (size_t)pl->g.G * 8, hipMemcpyDeviceToHost, s),
-                     "download");
-    if (st == PKDE_OK && r.chosen_out)
-      st = check_hip(hipMemcpyAsync(r.chosen_out, pl->d.chosen, (size_t)pl->g.G * 2, hipMemcpyDeviceToHost, s),
-                     "download");
-    if (st == PKDE_OK)
-      st = check_hip(hipMemcpyAsync(&errv[i], pl->d.err, 4, hipMemcpyDeviceToHost, s), "flag");
-    if (status_out) status_out[i] = st;
-    if (st != PKDE_OK && first_rc == PKDE_OK) first_rc = st;
+    if (status_out) status_out[i] = st[i];
+    if (st[i] != PKDE_OK && first_rc == PKDE_OK) first_rc = st[i];
   }
-  rc = check_hip(hipStreamSynchronize(s), "batch");
-  for (int i = 0; i < count; ++i)
-    if (status_out && status_out[i] == PKDE_OK && (errv[i] & 1)) {
-      status_out[i] = PKDE_OUT_OF_GRID;
-      if (first_rc == PKDE_OK) first_rc = fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid.");
-    }
-  for (auto& kv : plans) pkde_gradepro_plan_destroy(kv.second);
   return rc ? rc : first_rc;
 }
 
