@@ -154,13 +154,36 @@ def run_plain(args, wl_name):
         "ladder_ms": lad,
         "gp_rungs_per_s": U / (wall / args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": _measured_traffic(wl_name, args.precision),
+                     "traffic_unit": "bytes per estimate (ladder kernels)",
+                     "algorithmic_bytes": ladder_bytes,
                      "kernel": "ladder (axis-2 inverse pass + fused axis-1 inverse/VMR pass), "
                                "algorithmic bytes U*(2B+1)*(4|8) per estimate over the ladder's "
-                               "event-timed duration"},
+                               "event-timed duration; traffic = calibrated FETCH_SIZE+WRITE_SIZE "
+                               "bytes of the same kernels per estimate"},
     }
     plan.close()
     return out, wl
+
+
+def _measured_traffic(wl_name, precision):
+    """HBM bytes per estimate of the ladder kernels, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload (copy-kernel
+    calibrated); None when no such summary exists."""
+    files = {("c2", "fp64"): "c2_2d256_hbm_traffic.txt"}
+    name = files.get((wl_name, precision))
+    if name is None:
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("ladder_bytes_per_estimate:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def _box_draws(rng, n, d, lo, hi, kind, margin):

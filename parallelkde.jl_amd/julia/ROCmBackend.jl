@@ -121,8 +121,18 @@ struct PkdeParams
     precision::Int32
 end
 
-struct RocGradeProEstimator{N} <: DensityEstimators.AbstractGradeProEstimator{N,Float64,N + 1}
+# Mutable so that the finalizer is tied to the estimator object itself: the
+# plan is destroyed only once nothing references the estimator.
+mutable struct RocGradeProEstimator{N} <: DensityEstimators.AbstractGradeProEstimator{N,Float64,N + 1}
     plan::Ptr{Cvoid}
+end
+
+function destroy!(e::RocGradeProEstimator)
+    if e.plan != C_NULL
+        ccall((:pkde_gradepro_plan_destroy, libpkde), Cvoid, (Ptr{Cvoid},), e.plan)
+        e.plan = C_NULL
+    end
+    return nothing
 end
 
 function DensityEstimators.initialize_estimator(
@@ -164,16 +174,15 @@ function DensityEstimators.initialize_estimator(
     check(ccall((:pkde_gradepro_plan_create, libpkde), Cint,
         (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{PkdeParams}, Ref{Ptr{Cvoid}}),
         context(), N, n, lo, hi, h, p, plan))
-    est = RocGradeProEstimator{N}(plan[])
-    finalizer(e -> ccall((:pkde_gradepro_plan_destroy, libpkde), Cvoid, (Ptr{Cvoid},), e.plan), Ref(est))
-    return est
+    return finalizer(destroy!, RocGradeProEstimator{N}(plan[]))
 end
 
 # One ccall per estimate (GradeProEstimator.jl:968-1072 on the device).
 function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N}; method::Symbol, kwargs...) where {N}
     n = size(kde.data, 2)
     chosen = Array{Int16}(undef, size(kde.density))
-    GC.@preserve kde chosen begin
+    est.plan == C_NULL && throw(ArgumentError("estimator plan was destroyed"))
+    GC.@preserve est kde chosen begin
         check(ccall((:pkde_gradepro_estimate, libpkde), Cint,
             (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Int16}, Ptr{Cvoid}),
             est.plan, kde.data, n, C_NULL, kde.density, chosen, C_NULL))
