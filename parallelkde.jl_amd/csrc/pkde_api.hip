@@ -143,6 +143,15 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   if ((rc = dalloc((void**)&d.params, 256, "params"))) return rc;
   if ((rc = dalloc((void**)&d.mean, 256, "mean"))) return rc;
   if ((rc = dalloc((void**)&d.partial, (size_t)kMomentBlocks * PKDE_MAX_DIMS * 8, "partial"))) return rc;
+  if (!d.invk) {
+    // Welford weights 1 / (b + 1), b < B (IEEE division, as on the device)
+    std::vector<double> w(pl->B);
+    for (int b = 0; b < pl->B; ++b) w[b] = 1.0 / (double)(b + 1);
+    if ((rc = dalloc((void**)&d.invk, w.size() * sizeof(double), "welford weights"))) return rc;
+    if ((rc = check_hip(hipMemcpy(d.invk, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice),
+                        "welford weights")))
+      return rc;
+  }
   return PKDE_OK;
 }
 
@@ -150,7 +159,7 @@ static void plan_free(pkde_plan* pl) {
   PlanBuffers& d = pl->d;
   void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.chosen, d.sel, d.density, d.err,
                 d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
-                d.gM, d.gV};
+                d.gM, d.gV, d.invk};
   for (void* p : ps) dfree(p);
   d = PlanBuffers();
   for (auto& e : pl->ev)
@@ -224,7 +233,8 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
                                  tw, rows ? 1 : 0, s);
       }
       if (rows)
-        launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+        launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
+                              d.vmr, s);
       else
         launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
     }

@@ -215,22 +215,37 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One Stockham pass of radix R over the lane's own pencil `pb` (length L),
-// the lane being thread t of the pencil's L/E threads.  All lanes of the
-// wave call it (active = false for lanes without a pencil).
-template <int R, int E, int DIR, typename T>
-__device__ __forceinline__ void wave_pass(cplx<T>* pb, int L, int Ns, int t, bool active,
+// Wave-local pencils in LDS use a padded layout: element p of a pencil sits
+// at pad8(p) = p + p/8 (one spare slot after every 8 values), so the first
+// Stockham pass, whose lanes write R values apart, spreads its stores over
+// all banks.  Pencils of length L sit pencil_stride(L) values apart; the odd
+// remainder modulo 8 keeps accesses that walk across pencils conflict-free.
+__host__ __device__ constexpr int pad8(int p) { return p + (p >> 3); }
+__host__ __device__ constexpr int pencil_stride(int L) { return L + L / 8 + 1; }
+
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// One Stockham pass of radix R over the lane's own pencil `pb` (length
+// L = 2^LOG2L, padded layout), the lane being thread t of the pencil's L/E
+// threads; Ns = 2^LOG2NS is the length of the sub-transforms already done.
+// All sizes are compile-time, so index arithmetic reduces to shifts and masks.
+// All lanes of the wave call it (active = false for lanes without a pencil).
+template <int R, int E, int LOG2L, int LOG2NS, int DIR, typename T>
+__device__ __forceinline__ void wave_pass(cplx<T>* pb, int t, bool active,
                                           const cplx<T>* __restrict__ tw) {
-  const int tpp = L / E;
-  const int nbfly = L / R;
-  const int twstep = L / (Ns * R);
+  constexpr int L = 1 << LOG2L;
+  constexpr int tpp = L / E;
+  constexpr int nbfly = L / R;
+  constexpr int Ns = 1 << LOG2NS;
+  constexpr int LOG2R = ilog2c(R);
+  constexpr int twstep = L / (Ns * R);
   cplx<T> v[E];
   if (active) {
 #pragma unroll
     for (int u = 0; u < E / R; ++u) {
       const int j = t + u * tpp;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[u * R + r] = pb[j + r * nbfly];
+      for (int r = 0; r < R; ++r) v[u * R + r] = pb[pad8(j + r * nbfly)];
     }
   }
   wave_sync();
@@ -238,9 +253,9 @@ __device__ __forceinline__ void wave_pass(cplx<T>* pb, int L, int Ns, int t, boo
 #pragma unroll
     for (int u = 0; u < E / R; ++u) {
       const int j = t + u * tpp;
-      const int jm = j % Ns;
+      const int jm = j & (Ns - 1);
       cplx<T>* w = v + u * R;
-      if (jm != 0) {
+      if constexpr (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           cplx<T> f = tw[jm * r * twstep];
@@ -249,32 +264,37 @@ __device__ __forceinline__ void wave_pass(cplx<T>* pb, int L, int Ns, int t, boo
         }
       }
       dft_reg<R, DIR, T>(w);
-      const int out0 = (j / Ns) * Ns * R + jm;
+      const int out0 = ((j >> LOG2NS) << (LOG2NS + LOG2R)) + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) pb[out0 + r * Ns] = w[r];
+      for (int r = 0; r < R; ++r) pb[pad8(out0 + r * Ns)] = w[r];
     }
   }
   wave_sync();
 }
 
-// Wave-local FFT of the lane's pencil: L = 2^log2L with E <= L <= 64 * E,
-// so the L/E threads of a pencil sit in one wavefront.  E = 16 gives radix-16
-// passes; E = 8 halves the registers per lane (radix-8 passes), which the FP64
-// kernels need for occupancy.  Natural order in and out.
-template <int DIR, typename T, int E = 16>
-__device__ __forceinline__ void wave_fft(cplx<T>* pb, int log2L, int t, bool active,
-                                         const cplx<T>* __restrict__ tw) {
-  const int L = 1 << log2L;
-  constexpr int LE = E == 16 ? 4 : 3;
-  int Ns = 1, rem = log2L;
-  while (rem >= LE) {
-    wave_pass<E, E, DIR, T>(pb, L, Ns, t, active, tw);
-    Ns *= E;
-    rem -= LE;
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T>
+__device__ __forceinline__ void wave_fft_from(cplx<T>* pb, int t, bool active,
+                                              const cplx<T>* __restrict__ tw) {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  if constexpr (rem >= LE) {
+    wave_pass<E, E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw);
+    wave_fft_from<LOG2L, LOG2NS + LE, E, DIR, T>(pb, t, active, tw);
+  } else if constexpr (rem > 0) {
+    wave_pass<(1 << rem), E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw);
   }
-  if (rem == 3) wave_pass<8, E, DIR, T>(pb, L, Ns, t, active, tw);
-  else if (rem == 2) wave_pass<4, E, DIR, T>(pb, L, Ns, t, active, tw);
-  else if (rem == 1) wave_pass<2, E, DIR, T>(pb, L, Ns, t, active, tw);
+}
+
+// Wave-local FFT of the lane's pencil (padded layout): L = 2^LOG2L with
+// E <= L <= 64 * E, so the L/E threads of a pencil sit in one wavefront.
+// E = 16 gives radix-16 passes; E = 8 (radix-8 passes) halves the registers
+// per lane, which the FP64 kernels need for occupancy.  Natural order in and
+// out; `tw` is the table exp(-2 pi i k / L), k < L.
+template <int LOG2L, int DIR, typename T, int E>
+__device__ __forceinline__ void wave_fft(cplx<T>* pb, int t, bool active,
+                                         const cplx<T>* __restrict__ tw) {
+  static_assert((1 << LOG2L) >= E && (1 << LOG2L) <= 64 * E, "pencil must fit one wavefront");
+  wave_fft_from<LOG2L, 0, E, DIR, T>(pb, t, active, tw);
 }
 
 }  // namespace pkde

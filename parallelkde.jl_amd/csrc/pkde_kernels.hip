@@ -283,13 +283,14 @@ struct AxisArgs {
   int L, log2L;
   int64_t I, O, Q;
   int H1, H1p, TK, TI, ktiles, itiles;
+  int log2TK, log2TI, log2itiles, log2O;
   int transposed;
   const cplx<T>* tw;
 };
 
-template <typename T, int DIR, bool WAVE>
+// Generic form for pencil lengths outside the wave-local range.
+template <typename T, int DIR>
 __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
-  constexpr int E = sizeof(T) == 8 ? 8 : 16;  // elements per lane in the FFT
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = (cplx<T>*)smem;
   int64_t bid = blockIdx.x;
@@ -332,20 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
     buf[(kk * TI + ii) * L + e] = v;
   }
   __syncthreads();
-  if (WAVE) {
-    const int lp = L / E;  // lanes per pencil
-    const int lane = tid & 63, wv = tid >> 6;
-    const int ppw = 64 / lp;
-    const int nw = nthr >> 6;
-    for (int pb = wv * ppw; pb < npen; pb += nw * ppw) {
-      const int p = pb + lane / lp;
-      const bool act = p < npen;
-      wave_fft<DIR, T, E>(buf + (act ? p : 0) * L, a.log2L, lane % lp, act, a.tw + (L - 2));
-    }
-    __syncthreads();
-  } else {
-    lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
-  }
+  lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
   for (int id = tid; id < total; id += nthr) {
     int kk, ii, e;
     if (a.transposed) {
@@ -367,6 +355,115 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
       dst[q * a.H1p + k0 + kk] = v;
     else
       dst[(int64_t)(k0 + kk) * a.Q + q] = v;
+  }
+}
+
+// Wave-local form: pencils of compile-time length L = 2^LOG2L in the padded
+// LDS layout, one wave-wide FFT call per wave.  A block holds TK x TI
+// pencils (both powers of two; pencils with k1 beyond the Hermitian half are
+// skipped).  Index arithmetic is shifts and masks only.
+template <typename T, int DIR, int LOG2L>
+__global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
+  constexpr int E = sizeof(T) == 8 ? 8 : 16;  // elements per lane in the FFT
+  constexpr int L = 1 << LOG2L;
+  constexpr int PL = pencil_stride(L);
+  constexpr int lp = L / E;                    // lanes per pencil
+  constexpr int ppw = 64 / lp;                 // pencils per wave
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = (cplx<T>*)smem;
+  uint32_t bid = blockIdx.x;
+  const uint32_t it = bid & (uint32_t)(a.itiles - 1);
+  bid >>= a.log2itiles;
+  const uint32_t kt = bid % (uint32_t)a.ktiles;
+  bid /= (uint32_t)a.ktiles;
+  const uint32_t o = bid & (uint32_t)(a.O - 1);
+  bid >>= a.log2O;
+  const int fl = (int)(bid % (uint32_t)a.nf);
+  const int rr = (int)(bid / (uint32_t)a.nf);
+  const int f = a.fid0 + fl;
+  const int log2TI = a.log2TI, log2TK = a.log2TK;
+  const int TI = 1 << log2TI, TK = 1 << log2TK;
+  const int k0 = (int)kt * TK;
+  const int TKe = (a.H1 - k0) < TK ? (a.H1 - k0) : TK;
+  const int64_t i0 = (int64_t)it * TI;
+  const T* gt = nullptr;
+  if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize;
+  cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
+  const int64_t obase = (int64_t)o * L * a.I;
+  const int tid = threadIdx.x;
+  const int total = (TK * TI) << LOG2L;
+  // load: lanes run along the contiguous index (e if TI == 1, else i)
+  for (int id = tid; id < total; id += kThreads) {
+    const int ii = id & (TI - 1);
+    const int e = (id >> log2TI) & (L - 1);
+    const int kk = id >> (log2TI + LOG2L);
+    if (kk < TKe) {
+      const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+      cplx<T> v = src[(int64_t)(k0 + kk) * a.Q + q];
+      if (gt) v = cscale(v, gt[e]);
+      buf[(kk * TI + ii) * PL + pad8(e)] = v;
+    }
+  }
+  __syncthreads();
+  {
+    const int lane = tid & 63, wv = tid >> 6;
+    const int npen = TKe * TI;
+    for (int pb = wv * ppw; pb < TK * TI; pb += (kThreads / 64) * ppw) {
+      const int p = pb + lane / lp;
+      const bool act = p < npen;
+      wave_fft<LOG2L, DIR, T, E>(buf + (act ? p : 0) * PL, lane % lp, act, a.tw + (L - 2));
+    }
+  }
+  __syncthreads();
+  if (a.transposed) {
+    for (int id = tid; id < total; id += kThreads) {
+      const int kk = id & (TK - 1);
+      const int ii = (id >> log2TK) & (TI - 1);
+      const int e = id >> (log2TK + log2TI);
+      if (kk < TKe) {
+        const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+        dst[q * a.H1p + k0 + kk] = buf[(kk * TI + ii) * PL + pad8(e)];
+      }
+    }
+  } else {
+    for (int id = tid; id < total; id += kThreads) {
+      const int ii = id & (TI - 1);
+      const int e = (id >> log2TI) & (L - 1);
+      const int kk = id >> (log2TI + LOG2L);
+      if (kk < TKe) {
+        const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+        dst[(int64_t)(k0 + kk) * a.Q + q] = buf[(kk * TI + ii) * PL + pad8(e)];
+      }
+    }
+  }
+}
+
+template <typename T, int DIR>
+static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
+                             const AxisArgs<T>& a) {
+  if constexpr (sizeof(T) == 8) {
+    switch (log2L) {
+      case 3: k_axis_wave<T, DIR, 3><<<blocks, kThreads, lds, s>>>(a); break;
+      case 4: k_axis_wave<T, DIR, 4><<<blocks, kThreads, lds, s>>>(a); break;
+      case 5: k_axis_wave<T, DIR, 5><<<blocks, kThreads, lds, s>>>(a); break;
+      case 6: k_axis_wave<T, DIR, 6><<<blocks, kThreads, lds, s>>>(a); break;
+      case 7: k_axis_wave<T, DIR, 7><<<blocks, kThreads, lds, s>>>(a); break;
+      case 8: k_axis_wave<T, DIR, 8><<<blocks, kThreads, lds, s>>>(a); break;
+      case 9: k_axis_wave<T, DIR, 9><<<blocks, kThreads, lds, s>>>(a); break;
+      default: break;
+    }
+  } else {
+    switch (log2L) {
+      case 4: k_axis_wave<T, DIR, 4><<<blocks, kThreads, lds, s>>>(a); break;
+      case 5: k_axis_wave<T, DIR, 5><<<blocks, kThreads, lds, s>>>(a); break;
+      case 6: k_axis_wave<T, DIR, 6><<<blocks, kThreads, lds, s>>>(a); break;
+      case 7: k_axis_wave<T, DIR, 7><<<blocks, kThreads, lds, s>>>(a); break;
+      case 8: k_axis_wave<T, DIR, 8><<<blocks, kThreads, lds, s>>>(a); break;
+      case 9: k_axis_wave<T, DIR, 9><<<blocks, kThreads, lds, s>>>(a); break;
+      case 10: k_axis_wave<T, DIR, 10><<<blocks, kThreads, lds, s>>>(a); break;
+      default: break;
+    }
   }
 }
 
@@ -409,6 +506,25 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   // pencils per workgroup: one per L/E lanes (E = 8 in FP64, 16 in FP32)
   const int Ef = sizeof(TI) == 8 ? 8 : 16;
   const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
+  a.transposed = transposed;
+  a.tw = (const cplx<TI>*)tw;
+  if (a.L >= Ef && a.L <= 64 * Ef) {
+    // wave-local form: TI and TK powers of two, TK * TI <= per_block
+    a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, std::min(8, per_block));
+    int hp = 1;
+    while (hp < g.H1) hp <<= 1;
+    a.TK = std::min(hp, per_block / a.TI);
+    a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+    a.itiles = (int)(a.I / a.TI);
+    a.log2TI = ilog2c(a.TI);
+    a.log2TK = ilog2c(a.TK);
+    a.log2itiles = ilog2c(a.itiles);
+    a.log2O = ilog2c((int)a.O);
+    const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
+    const size_t lds = (size_t)a.TK * a.TI * pencil_stride(a.L) * sizeof(cplx<TI>);
+    axis_wave_launch<TI, DIR>(a.log2L, (unsigned)blocks, lds, s, a);
+    return;
+  }
   if (a.I == 1) {
     a.TI = 1;
   } else {
@@ -417,14 +533,9 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.TK = std::max(1, std::min(g.H1, per_block / a.TI));
   a.ktiles = (g.H1 + a.TK - 1) / a.TK;
   a.itiles = (int)(a.I / a.TI);
-  a.transposed = transposed;
-  a.tw = (const cplx<TI>*)tw;
   const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
   const size_t lds = (size_t)a.TK * a.TI * a.L * sizeof(cplx<TI>);
-  if (a.L >= Ef && a.L <= 64 * Ef)
-    k_axis_fft<TI, DIR, true><<<(unsigned)blocks, kThreads, lds, s>>>(a);
-  else
-    k_axis_fft<TI, DIR, false><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  k_axis_fft<TI, DIR><<<(unsigned)blocks, kThreads, lds, s>>>(a);
 }
 
 #define PKDE_AXIS_INST(TI, TO, DIR)                                                            \
@@ -499,6 +610,7 @@ struct VmrArgs {
   int64_t G, Q;
   int n1, log2n1, H1, TQ, NBP, ntiles;
   int64_t rowp;  // row stride of the transposed layout (rows kernel)
+  const double* invk;  // 1 / (b + 1), b < B (Welford weights)
 };
 
 template <typename T, int PPT>
@@ -658,17 +770,21 @@ template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int
 // per row in its own LDS slice, transforms them with wave-local passes and
 // accumulates the FP64 Welford statistics of its 8 points per lane; the
 // wavefronts of a workgroup never synchronise with each other.
-template <typename T, int E, int PPT>
+template <typename T, int E, int PPT, int LOG2N1>
 __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
+  constexpr int n1 = 1 << LOG2N1;
+  constexpr int H = n1 / 2;                // Hermitian half without the Nyquist term
+  constexpr int W = 64 * PPT / n1;         // rows per wave (PPT points per lane)
+  constexpr int LOG2W = ilog2c(W);
+  constexpr int PL = pencil_stride(n1);
+  constexpr int lp = n1 / E;               // lanes per pencil
+  constexpr int ppc = 64 / lp;             // pencils per wave-wide FFT call
+  static_assert(W >= 1 && (W & (W - 1)) == 0, "rows per wave must be a power of two");
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int n1 = a.n1, H1 = a.H1, B = a.B;
-  const int W = 64 * PPT / n1;       // rows per wave (PPT points per lane)
-  const int lp = n1 / E;             // lanes per pencil
-  const int ppc = 64 / lp;           // pencils per wave-wide FFT call
-  cplx<T>* Z = (cplx<T>*)smem + (int64_t)wv * 2 * W * n1;
-  const int64_t rows_per_rung = a.Q / W;
+  cplx<T>* Z = (cplx<T>*)smem + wv * (2 * W * PL);
+  const int64_t rows_per_rung = a.Q >> LOG2W;
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   if (gw >= (int64_t)a.R * rows_per_rung) return;
   const int rr = (int)(gw / rows_per_rung);
@@ -676,8 +792,24 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   const int r = a.r0 + rr;
   const T* gm1 = a.gM + (int64_t)r * a.ksum;
   const T* gv1 = a.gV + (int64_t)r * a.ksum;
-  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + q0 * a.rowp;
   const double vfac = a.cV[r] * a.invG;
+
+  // Per lane: PPT (row, k1) items of the half spectrum k1 < n1/2, plus the
+  // Nyquist term k1 = n1/2 of row `lane` for lanes < 2W.  Row index
+  // ty * W + w: ty = 0 sums field, ty = 1 squares field.
+  T ld_g[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int id = lane + 64 * u;
+    const int k1 = id & (H - 1);
+    ld_g[u] = (id >> (LOG2N1 - 1 + LOG2W)) ? gv1[k1] : gm1[k1];
+  }
+  const bool nyq = lane < 2 * W;
+  const int nrow = nyq ? lane : 0;
+  const int nty = nrow >> LOG2W, nw = nrow & (W - 1);
+  const T ng = nty ? gv1[H] : gm1[H];
+  const int nyq_off = nw * (int)a.rowp + H;
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -685,34 +817,48 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     mean[u] = 0.0;
     m2[u] = 0.0;
   }
-  const int nload = 2 * W * H1;
+  const int B = a.B;
   for (int b = 0; b < B; b += 2) {
     const bool has2 = b + 1 < B;
-    for (int id = lane; id < nload; id += 64) {
-      const int k1 = id % H1;
-      const int rest = id / H1;
-      const int w = rest % W, ty = rest / W;
-      const int fa = ty * B + b;
-      const T gk = ty ? gv1[k1] : gm1[k1];
-      const int64_t o = (q0 + w) * a.rowp + k1;
-      const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + o], gk);
+    // field b of the sums (ty = 0) or squares (ty = 1) block
+    const cplx<T>* f0 = src + (int64_t)b * a.fsize;
+    const cplx<T>* fS = src + (int64_t)(B + b) * a.fsize;
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int id = lane + 64 * u;
+      const int k1 = id & (H - 1);
+      const int rw = id >> (LOG2N1 - 1);
+      const int off = (rw & (W - 1)) * (int)a.rowp + k1;
+      const cplx<T>* fb = (rw >> LOG2W) ? fS : f0;
+      const cplx<T> A = cscale(fb[off], ld_g[u]);
       cplx<T> Bv = {(T)0, (T)0};
-      if (has2) Bv = cscale(src[(int64_t)(fa + 1) * a.fsize + o], gk);
-      put_pair(Z + (ty * W + w) * n1, n1, k1, A, Bv);
+      if (has2) Bv = cscale(fb[a.fsize + off], ld_g[u]);
+      cplx<T>* row = Z + rw * PL;
+      row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+      if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+    }
+    if (nyq) {
+      const cplx<T>* fb = nty ? fS : f0;
+      const cplx<T> A = cscale(fb[nyq_off], ng);
+      cplx<T> Bv = {(T)0, (T)0};
+      if (has2) Bv = cscale(fb[a.fsize + nyq_off], ng);
+      Z[nrow * PL + pad8(H)] = {A.re - Bv.im, A.im + Bv.re};
     }
     wave_sync();
+#pragma unroll
     for (int c = 0; c < 2 * W; c += ppc) {
       const int p = c + lane / lp;
       const bool act = p < 2 * W;
-      wave_fft<1, T, E>(Z + (act ? p : 0) * n1, a.log2n1, lane % lp, act, a.tw + (n1 - 2));
+      wave_fft<LOG2N1, 1, T, E>(Z + (act ? p : 0) * PL, lane % lp, act, a.tw + (n1 - 2));
     }
-    const double inv1 = 1.0 / (double)(b + 1), inv2 = 1.0 / (double)(b + 2);
+    const double inv1 = a.invk[b];
+    const double inv2 = has2 ? a.invk[b + 1] : 0.0;
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int pt = lane + 64 * u;
-      const int w = pt / n1, j1 = pt - w * n1;
-      const cplx<T> zm = Z[w * n1 + j1];
-      const cplx<T> zv = Z[(W + w) * n1 + j1];
+      const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
+      const cplx<T> zm = Z[w * PL + pad8(j1)];
+      const cplx<T> zv = Z[(W + w) * PL + pad8(j1)];
       {
         const double am = fabs((double)zm.re) * a.invG;
         const double av = fabs((double)zv.re) * vfac;
@@ -736,19 +882,42 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int pt = lane + 64 * u;
-    const int w = pt / n1, j1 = pt - w * n1;
+    const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
     const double vv = sc * m2[u] / (double)(B - 1);
     a.vmr[(int64_t)r * a.G + (q0 + w) * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
   }
 }
 
-// Points per lane: 4 in FP64 (8 KB of LDS and ~half the registers of 8
-// points per wave), 8 in FP32; a 512-point row needs 8.
+// Points per lane: 4 in FP64 for rows up to 256 points (8 KB of LDS per wave
+// and about half the registers of 8 points), 8 otherwise; a 512-point row
+// needs 8.
+template <typename T, int E, int PPT, int LOG2N1>
+static void rows_launch(unsigned blocks, size_t lds, hipStream_t s, const VmrArgs<T>& a) {
+  k_ladder_rows<T, E, PPT, LOG2N1><<<blocks, kThreads, lds, s>>>(a);
+}
+
+// Row lengths 16..512 with W = 64 * PPT / n1 >= 1 rows per wave.
+template <typename T, int E, int PPT>
+static void rows_dispatch(int log2n1, unsigned blocks, size_t lds, hipStream_t s,
+                          const VmrArgs<T>& a) {
+  switch (log2n1) {
+    case 4: rows_launch<T, E, PPT, 4>(blocks, lds, s, a); break;
+    case 5: rows_launch<T, E, PPT, 5>(blocks, lds, s, a); break;
+    case 6: rows_launch<T, E, PPT, 6>(blocks, lds, s, a); break;
+    case 7: rows_launch<T, E, PPT, 7>(blocks, lds, s, a); break;
+    case 8: rows_launch<T, E, PPT, 8>(blocks, lds, s, a); break;
+    case 9:
+      if constexpr (PPT == 8) rows_launch<T, E, PPT, 9>(blocks, lds, s, a);
+      break;
+    default: break;
+  }
+}
+
 template <typename T>
 void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
-                        const double* cV, const double* vscale, const T* tw, double* vmr,
-                        hipStream_t s) {
+                        const double* cV, const double* vscale, const T* tw, const double* invk,
+                        double* vmr, hipStream_t s) {
   VmrArgs<T> a;
   const int n1 = g.n[0];
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -769,6 +938,7 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   a.cV = cV;
   a.vscale = vscale;
   a.tw = (const cplx<T>*)tw;
+  a.invk = invk;
   a.vmr = vmr;
   a.G = g.G;
   a.Q = g.Q;
@@ -781,20 +951,21 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   const int64_t waves = (int64_t)R * (g.Q / W);
   const int wpb = kThreads / 64;
   const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
-  const size_t lds = (size_t)wpb * 2 * W * n1 * sizeof(cplx<T>);
+  const size_t lds = (size_t)wpb * 2 * W * pencil_stride(n1) * sizeof(cplx<T>);
   if (ppt == 4)
-    k_ladder_rows<T, E, 4><<<blocks, kThreads, lds, s>>>(a);
+    rows_dispatch<T, E, 4>(g.log2n[0], blocks, lds, s, a);
   else
-    k_ladder_rows<T, E, 8><<<blocks, kThreads, lds, s>>>(a);
+    rows_dispatch<T, E, 8>(g.log2n[0], blocks, lds, s, a);
 }
 
 template void launch_ladder_rows<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t,
                                         int, int, const float*, const float*, const double*,
-                                        const double*, const float*, double*, hipStream_t);
+                                        const double*, const float*, const double*, double*,
+                                        hipStream_t);
 template void launch_ladder_rows<double>(const GridDev&, const cplx<double>*, int64_t, int,
                                          int64_t, int, int, const double*, const double*,
-                                         const double*, const double*, const double*, double*,
-                                         hipStream_t);
+                                         const double*, const double*, const double*,
+                                         const double*, double*, hipStream_t);
 
 // -------------------------------------------------------------- convergence
 // find_stability (DirectSpace.jl:975-1031) applied rung after rung with the

@@ -64,8 +64,8 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
 template <typename T>
 void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
-                        const double* cV, const double* vscale, const T* tw, double* vmr,
-                        hipStream_t s);
+                        const double* cV, const double* vscale, const T* tw, const double* invk,
+                        double* vmr, hipStream_t s);
 
 // Convergence state machine over all rungs -> chosen rung, selection rung.
 void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,

@@ -65,6 +65,7 @@ struct PlanBuffers {
   double* params = nullptr;
   void* gM = nullptr;
   void* gV = nullptr;
+  double* invk = nullptr;  // Welford weights 1 / (b + 1)
   int S_alloc = 0;
 };
 
