@@ -223,15 +223,15 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     cplx<T>* Y2 = Y + cap;
     for (int r0 = 0; r0 < S; r0 += Rc) {
       const int R = std::min(Rc, S - r0);
-      if (N == 2) {
-        launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
-                                 rows ? 1 : 0, s);
-      } else {
-        launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y2, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
-                                 0, s);
+      // first pass (grid axis 2): spectra read once per chunk
+      cplx<T>* Y1 = N == 2 ? Y : Y2;
+      const int tr1 = (N == 2 && rows) ? 1 : 0;
+      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, tr1, s))
+        launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
+                                 tr1, s);
+      if (N == 3)
         launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B,
                                  tw, rows ? 1 : 0, s);
-      }
       if (rows)
         launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
                               d.vmr, s);

@@ -439,6 +439,178 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   }
 }
 
+// Ladder form of the first inverse axis pass (axis 2, contiguous, TI = 1):
+// the block keeps its TK spectrum columns of one field in registers -- each
+// lane holds exactly the inputs of its first Stockham pass -- and loops over
+// the R rungs of the chunk, so the spectra are read once per chunk rather
+// than once per rung.  Per rung: Gaussian factor, first pass from registers,
+// remaining passes through LDS, store of the block's pencils (transposed
+// [q][k1] rows for the row kernel, or the [k1][q] layout for a further pass).
+// The arithmetic is that of k_axis_wave with DIR = +1, operation for operation.
+template <typename T, int LOG2L>
+__global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
+  constexpr int E = sizeof(T) == 8 ? 8 : 16;
+  constexpr int L = 1 << LOG2L;
+  constexpr int PL = pencil_stride(L);
+  constexpr int lp = L / E;                  // lanes per pencil
+  constexpr int ppw = 64 / lp;               // pencils per wave
+  constexpr int LE = ilog2c(E);
+  constexpr int R1 = LOG2L >= LE ? E : L;    // radix of the first pass
+  constexpr int LOG2R1 = ilog2c(R1);
+  constexpr int nb1 = L / R1;
+  constexpr int tpp = L / E;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = (cplx<T>*)smem;
+  uint32_t bid = blockIdx.x;
+  const uint32_t kt = bid % (uint32_t)a.ktiles;
+  bid /= (uint32_t)a.ktiles;
+  const uint32_t o = bid & (uint32_t)(a.O - 1);
+  bid >>= a.log2O;
+  const int f = a.fid0 + (int)bid;
+  const int log2TK = a.log2TK;
+  const int TK = 1 << log2TK;
+  const int k0 = (int)kt * TK;
+  const int TKe = (a.H1 - k0) < TK ? (a.H1 - k0) : TK;
+  const int64_t obase = (int64_t)o * L;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int p = wv * ppw + lane / lp;   // this lane's pencil (column k0 + p)
+  const int t = lane % lp;
+  const bool act = p < TKe;
+  const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
+  const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
+  cplx<T> x[E];
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < E / R1; ++u)
+#pragma unroll
+      for (int r = 0; r < R1; ++r)
+        x[u * R1 + r] = src[(int64_t)(k0 + p) * a.Q + obase + t + u * tpp + r * nb1];
+  }
+  cplx<T>* pb = buf + (act ? p : 0) * PL;
+  const int total = TK << LOG2L;
+  for (int rr = 0; rr < a.R; ++rr) {
+    const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
+    if (act) {
+#pragma unroll
+      for (int u = 0; u < E / R1; ++u) {
+        const int j = t + u * tpp;
+        cplx<T> w[R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) w[r] = cscale(x[u * R1 + r], gt[j + r * nb1]);
+        dft_reg<R1, 1, T>(w);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
+      }
+    }
+    wave_sync();
+    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
+    __syncthreads();
+    cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
+    if (a.transposed) {
+      for (int id = tid; id < total; id += kThreads) {
+        const int kk = id & (TK - 1);
+        const int e = id >> log2TK;
+        if (kk < TKe) dst[(obase + e) * a.H1p + k0 + kk] = buf[kk * PL + pad8(e)];
+      }
+    } else {
+      for (int id = tid; id < total; id += kThreads) {
+        const int e = id & (L - 1);
+        const int kk = id >> LOG2L;
+        if (kk < TKe) dst[(int64_t)(k0 + kk) * a.Q + obase + e] = buf[kk * PL + pad8(e)];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
+                               const AxisArgs<T>& a) {
+  if constexpr (sizeof(T) == 8) {
+    switch (log2L) {
+      case 3: k_axis_ladder<T, 3><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 4: k_axis_ladder<T, 4><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 5: k_axis_ladder<T, 5><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 6: k_axis_ladder<T, 6><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 7: k_axis_ladder<T, 7><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 8: k_axis_ladder<T, 8><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 9: k_axis_ladder<T, 9><<<blocks, kThreads, lds, s>>>(a); return true;
+      default: return false;
+    }
+  } else {
+    switch (log2L) {
+      case 4: k_axis_ladder<T, 4><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 5: k_axis_ladder<T, 5><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 6: k_axis_ladder<T, 6><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 7: k_axis_ladder<T, 7><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 8: k_axis_ladder<T, 8><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 9: k_axis_ladder<T, 9><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 10: k_axis_ladder<T, 10><<<blocks, kThreads, lds, s>>>(a); return true;
+      default: return false;
+    }
+  }
+}
+
+// First inverse pass of a rung chunk along grid axis 2 for the 2B bootstrap
+// fields (fields [0, B) use the means factors, [B, 2B) the variance factors):
+// spec [field][k1][q] -> dst rungs [r0, r0 + R).  Returns false when the axis
+// length is outside the wave-local range (the caller then uses the per-rung
+// pass).
+template <typename T>
+bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
+                        int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
+                        const T* gV, int fsplit, const T* tw, int transposed, hipStream_t s) {
+  AxisArgs<T> a{};
+  const int Ef = sizeof(T) == 8 ? 8 : 16;
+  a.L = g.n[1];
+  a.log2L = g.log2n[1];
+  if (a.L < Ef || a.L > 64 * Ef) return false;
+  a.src = spec;
+  a.dst = dst;
+  a.src_rs = 0;
+  a.dst_rs = dst_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.dst_fsize = transposed ? (int64_t)g.Q * g.H1p : a.fsize;
+  a.fid0 = 0;
+  a.nf = nf;
+  a.src_fid0 = 0;
+  a.dst_fid0 = 0;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.fsplit = fsplit;
+  a.ksum = g.ksum;
+  a.goff = g.koff[1];
+  a.Q = g.Q;
+  a.H1 = g.H1;
+  a.H1p = g.H1p;
+  a.I = 1;
+  a.O = g.n[2];
+  a.TI = 1;
+  const int per_block = (kThreads * Ef) / a.L;
+  int hp = 1;
+  while (hp < g.H1) hp <<= 1;
+  a.TK = std::min(hp, per_block);
+  a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+  a.itiles = 1;
+  a.log2TI = 0;
+  a.log2TK = ilog2c(a.TK);
+  a.log2itiles = 0;
+  a.log2O = ilog2c((int)a.O);
+  a.transposed = transposed;
+  a.tw = (const cplx<T>*)tw;
+  const int64_t blocks = (int64_t)nf * a.O * a.ktiles;
+  const size_t lds = (size_t)a.TK * pencil_stride(a.L) * sizeof(cplx<T>);
+  return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
+}
+template bool launch_axis_ladder<double>(const GridDev&, const cplx<double>*, cplx<double>*,
+                                         int64_t, int, int, int, const double*, const double*,
+                                         int, const double*, int, hipStream_t);
+template bool launch_axis_ladder<float>(const GridDev&, const cplx<float>*, cplx<float>*,
+                                        int64_t, int, int, int, const float*, const float*, int,
+                                        const float*, int, hipStream_t);
+
 template <typename T, int DIR>
 static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
                              const AxisArgs<T>& a) {

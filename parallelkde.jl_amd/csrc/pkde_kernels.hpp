@@ -48,6 +48,14 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
                      int transposed, hipStream_t s);
 
+// Ladder form of the first inverse pass along grid axis 2: spectra read once
+// per chunk of R rungs (see k_axis_ladder).  False if the axis length is
+// outside the wave-local range.
+template <typename T>
+bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
+                        int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
+                        const T* gV, int fsplit, const T* tw, int transposed, hipStream_t s);
+
 template <typename TI, typename TO>
 void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream_t s);
 
