@@ -92,6 +92,7 @@ uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
   b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
   if (pl->g.N >= 2) b += (uint64_t)y_units(pl) * y_part_bytes(pl);
+  if (pl->fixed) b += binsort_bytes(pl->g, n, pl->B);  // sorted-binning workspace
   return b;
 }
 
@@ -143,6 +144,16 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   if ((rc = dalloc((void**)&d.params, 256, "params"))) return rc;
   if ((rc = dalloc((void**)&d.mean, 256, "mean"))) return rc;
   if ((rc = dalloc((void**)&d.partial, (size_t)kMomentBlocks * PKDE_MAX_DIMS * 8, "partial"))) return rc;
+  if (pl->fixed) {
+    const uint64_t need = binsort_bytes(pl->g, n, pl->B);
+    if (need > d.binwork_bytes) {
+      dfree(d.binwork);
+      d.binwork = nullptr;
+      d.binwork_bytes = 0;
+      if ((rc = dalloc(&d.binwork, need, "binning workspace"))) return rc;
+      d.binwork_bytes = need;
+    }
+  }
   if (!d.invk) {
     // Welford weights 1 / (b + 1), b < B (IEEE division, as on the device)
     std::vector<double> w(pl->B);
@@ -159,7 +170,7 @@ static void plan_free(pkde_plan* pl) {
   PlanBuffers& d = pl->d;
   void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.chosen, d.sel, d.density, d.err,
                 d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
-                d.gM, d.gV, d.invk};
+                d.gM, d.gV, d.invk, d.binwork};
   for (void* p : ps) dfree(p);
   d = PlanBuffers();
   for (auto& e : pl->ev)
@@ -177,7 +188,9 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
 
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
   launch_fill_i32(d.err, 0, 1, s);
-  launch_zero_bytes(d.acc, F * G * 8, s);
+  // the sorted fixed-point binning writes every accumulator cell itself
+  const bool sorted = pl->fixed && d.binwork;
+  if (!sorted) launch_zero_bytes(d.acc, F * G * 8, s);
   if (!pl->bw_given) {
     const double pref = pow(4.0 / ((double)(N + 2) * (double)n), 1.0 / (double)(N + 4));
     launch_silverman(N, d_x, n, pref, d.partial, d.mean, d.bw, s);
@@ -185,7 +198,14 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   launch_tables<T>(g, S, pl->steps_mode, pl->steps_or_step, d.bw, d.kfreq, n, pl->alpha_s,
                    pl->alpha_os, pl->eps, d.times, d.dlogt, d.vscale, d.cV, d.params, (T*)d.gM,
                    (T*)d.gV, s);
-  launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err, s);
+  if (sorted) {
+    if (launch_binning_sorted(g, d_x, n, d_idx, B, pl->p.rng_seed, frac_bits_for(n), d.acc, d.err,
+                              d.binwork, d.binwork_bytes, s))
+      return fail(PKDE_DEVICE_ERROR, "binning workspace does not fit the estimate");
+  } else {
+    launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err,
+                   s);
+  }
   {
     double hp = g.h[0];
     for (int i = 1; i < N; ++i) hp *= g.h[i];
@@ -647,7 +667,10 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
   int32_t* di = nullptr;
   void* acc = nullptr;
   int* derr = nullptr;
+  void* work = nullptr;
+  const uint64_t wbytes = deterministic ? binsort_bytes(g, n, B) : 0;
   int rc = dalloc((void**)&dx, (size_t)n * ndims * 8, "samples");
+  if (!rc && wbytes) rc = dalloc(&work, wbytes, "binning workspace");
   if (!rc) rc = dalloc((void**)&di, (size_t)n * B * 4, "indices");
   if (!rc) rc = dalloc(&acc, (size_t)F * G * 8, "acc");
   if (!rc) rc = dalloc((void**)&derr, 256, "err");
@@ -657,9 +680,12 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
   if (!rc) {
     hipStream_t s = ctx->stream;
     launch_fill_i32(derr, 0, 1, s);
-    launch_zero_bytes(acc, F * G * 8, s);
     const int fb = frac_bits_for(n);
-    launch_binning(g, dx, n, di, B, 0, deterministic, fb, acc, derr, s);
+    if (!deterministic ||
+        launch_binning_sorted(g, dx, n, di, B, 0, fb, acc, derr, work, wbytes, s)) {
+      launch_zero_bytes(acc, F * G * 8, s);
+      launch_binning(g, dx, n, di, B, 0, deterministic, fb, acc, derr, s);
+    }
     rc = check_hip(hipStreamSynchronize(s), "binning");
     if (!rc) rc = check_hip(hipMemcpy(&err, derr, 4, hipMemcpyDeviceToHost), "flag");
     if (!rc) {
@@ -680,7 +706,7 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
       }
     }
   }
-  dfree(dx); dfree(di); dfree(acc); dfree(derr);
+  dfree(dx); dfree(di); dfree(acc); dfree(derr); dfree(work);
   if (rc) return rc;
   if (err & 1) return fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid. Remove them or increase grid boundaries.");
   if (err & 2) return fail(PKDE_INVALID_ARGUMENT, "bootstrap index outside 0..n-1");

@@ -34,6 +34,16 @@ void launch_binning(const GridDev& g, const double* x, int64_t n, const int32_t*
                     uint64_t seed, int fixed, int frac_bits, void* acc, int* err,
                     hipStream_t s);
 
+// Deterministic fixed-point binning without global atomics (counting sort of
+// the samples by cell, per-bootstrap multiplicities, per-tile LDS deposits);
+// writes every accumulator cell, so acc needs no zeroing.  Same integers as
+// launch_binning with fixed != 0.  `work` must hold binsort_bytes(); returns
+// nonzero (nothing launched) if it does not or the sizes are out of range.
+uint64_t binsort_bytes(const GridDev& g, int64_t n, int B);
+int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
+                          uint64_t seed, int frac_bits, void* acc, int* err, void* work,
+                          uint64_t work_bytes, hipStream_t s);
+
 // acc -> forward transform along axis 1 in FP64, Hermitian halves written to
 // specD[fid][k1][q].
 void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double scaleM,

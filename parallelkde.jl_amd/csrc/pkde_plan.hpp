@@ -66,6 +66,8 @@ struct PlanBuffers {
   void* gM = nullptr;
   void* gV = nullptr;
   double* invk = nullptr;  // Welford weights 1 / (b + 1)
+  void* binwork = nullptr;  // workspace of the sorted binning
+  uint64_t binwork_bytes = 0;
   int S_alloc = 0;
 };
 

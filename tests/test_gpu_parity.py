@@ -116,6 +116,32 @@ def test_binning_cell_edges_bit_exact(L, pk, n_dims):
     assert np.all(es[s > 0] > 0)
 
 
+@pytest.mark.parametrize("n_dims", [1, 2])
+def test_binning_multiplicity_overflow(L, pk, n_dims):
+    """Fixed-point binning counts how often each sample is drawn per
+    resample in 8 bits; a resample drawing one sample 300 times overflows the
+    count and is binned draw by draw instead.  Both resamples must match the
+    oracle and be identical to a draw-by-draw binning of the same draws."""
+    say(f"[binning overflow] N={n_dims}")
+    rng = np.random.default_rng(40 + n_dims)
+    g = O.make_grid([-4.0] * n_dims, [4.0] * n_dims, [64] * n_dims)
+    n = 300
+    x = np.clip(rng.standard_normal((n, n_dims)), -3.5, 3.5)
+    idx = np.vstack([np.full(n, 17, dtype=np.int32),            # overflows
+                     rng.integers(0, n, n).astype(np.int32)])    # ordinary
+    _, es, eq = O.dirac_sequences(g, x, idx)
+    rc, s, q = _gpu_dirac(L, pk, g, x, idx, 1)
+    assert rc == 0
+    hp = np.prod([g.h[i] for i in range(n_dims)])
+    qM = 2.0 ** -40 / (n * hp)
+    qV = 2.0 ** -40 / (n * hp) ** 2
+    assert np.abs(s - es).max() <= 0.5 * qM * n * 2 ** n_dims
+    assert np.abs(q - eq).max() <= 0.5 * qV * n * 2 ** n_dims
+    # the overflowing resample carries the full mass: n deposits of 1/(n h^N)
+    assert abs(s[0].sum() * hp - 1.0) < 1e-9
+    assert np.count_nonzero(s[0]) <= 2 ** n_dims
+
+
 def test_binning_out_of_grid(L, pk):
     g = O.make_grid([-1.0], [1.0], [32])
     x = np.array([[0.0], [1.0]])
