@@ -72,12 +72,18 @@ static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; 
 // rung of the 2B bootstrap fields.
 static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
 
+// Rungs per chunk: as many as the chunk budget holds, and -- unless the
+// budget was set explicitly -- at least 4 when they fit in 1/8 of the device
+// memory (the first axis pass reads the spectra once per chunk).
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.Q * pl->g.H1p * 2 * rs;
-  uint64_t part = std::min<uint64_t>((uint64_t)pl->S * per_rung,
-                                     std::max<uint64_t>(per_rung, pl->ctx->chunk_bytes / y_units(pl)));
-  return std::max<uint64_t>(part, (uint64_t)fsize(pl) * 2 * rs * 2);
+  const uint64_t units = (uint64_t)y_units(pl);
+  uint64_t rungs = std::max<uint64_t>(1, pl->ctx->chunk_bytes / units / per_rung);
+  if (!pl->ctx->chunk_fixed)
+    rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
+  rungs = std::min<uint64_t>(rungs, (uint64_t)pl->S);
+  return std::max<uint64_t>(rungs * per_rung, (uint64_t)fsize(pl) * 2 * rs * 2);
 }
 
 uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
@@ -236,6 +242,9 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     // row kernel reads; in 3-D the axis-2 result goes through a second buffer
     const int64_t per_rung = 2 * (int64_t)B * g.Q * g.H1p;  // padded rows
     const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512;
+    const int Ew = sizeof(T) == 8 ? 8 : 16;  // wave-local FFT range [E, 64 E]
+    const bool rows3 = N == 3 && rows && g.n[1] >= Ew && g.n[1] <= 64 * Ew && g.n[2] >= Ew &&
+                       g.n[2] <= 64 * Ew;
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
     const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
@@ -243,15 +252,21 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     cplx<T>* Y2 = Y + cap;
     for (int r0 = 0; r0 < S; r0 += Rc) {
       const int R = std::min(Rc, S - r0);
-      // first pass (grid axis 2): spectra read once per chunk
+      // first pass (grid axis 2): spectra read once per chunk.  3-D with the
+      // row kernel: the first pass leaves axis 3 contiguous ([k1][j2][j3]),
+      // so the axis-3 pass reads whole pencils and writes [q][k1] rows.
       cplx<T>* Y1 = N == 2 ? Y : Y2;
-      const int tr1 = (N == 2 && rows) ? 1 : 0;
-      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, tr1, s))
+      const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
+      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s))
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
-                                 tr1, s);
-      if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B,
-                                 tw, rows ? 1 : 0, s);
+                                 mode1 == 1 ? 1 : 0, s);
+      if (N == 3) {
+        if (!rows3)
+          launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV,
+                                   B, tw, rows ? 1 : 0, s);
+        else
+          launch_axis3_rows<T>(g, Y2, per_rung, Y, per_rung, 2 * B, R, r0, gM, gV, B, tw, s);
+      }
       if (rows)
         launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
                               d.vmr, s);
@@ -355,7 +370,14 @@ int pkde_context_create(int device, pkde_ctx** out) {
     pkde_context_destroy(c);
     return rc;
   }
-  if (const char* e = getenv("PKDE_CHUNK_BYTES")) c->chunk_bytes = strtoull(e, nullptr, 10);
+  if (const char* e = getenv("PKDE_CHUNK_BYTES")) {
+    c->chunk_bytes = strtoull(e, nullptr, 10);
+    c->chunk_fixed = true;
+  }
+  {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->mem_total = tot;
+  }
   *out = c;
   return PKDE_OK;
 }

@@ -285,6 +285,12 @@ struct AxisArgs {
   int H1, H1p, TK, TI, ktiles, itiles;
   int log2TK, log2TI, log2itiles, log2O;
   int transposed;
+  // output pencil placement of the transposed store: q_out = o * oO + e * oE + i * oI
+  // (o: outer pencil index, e: position along the axis, i: inner index)
+  int64_t oO, oE, oI;
+  // ladder pass: TJ outer pencils per block, store mode 0 [k1][q], 1 [q][k1],
+  // 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass)
+  int TJ, log2TJ, log2otiles, mode;
   const cplx<T>* tw;
 };
 
@@ -422,7 +428,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
       const int ii = (id >> log2TK) & (TI - 1);
       const int e = id >> (log2TK + log2TI);
       if (kk < TKe) {
-        const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+        const int64_t q = (int64_t)o * a.oO + (int64_t)e * a.oE + (i0 + ii) * a.oI;
         dst[q * a.H1p + k0 + kk] = buf[(kk * TI + ii) * PL + pad8(e)];
       }
     }
@@ -439,14 +445,15 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   }
 }
 
-// Ladder form of the first inverse axis pass (axis 2, contiguous, TI = 1):
-// the block keeps its TK spectrum columns of one field in registers -- each
-// lane holds exactly the inputs of its first Stockham pass -- and loops over
-// the R rungs of the chunk, so the spectra are read once per chunk rather
-// than once per rung.  Per rung: Gaussian factor, first pass from registers,
-// remaining passes through LDS, store of the block's pencils (transposed
-// [q][k1] rows for the row kernel, or the [k1][q] layout for a further pass).
-// The arithmetic is that of k_axis_wave with DIR = +1, operation for operation.
+// Ladder form of the first inverse axis pass (grid axis 2, contiguous): the
+// block keeps TK x TJ spectrum pencils (TK columns k1, TJ outer indices j3)
+// of one field in registers -- each lane holds exactly the inputs of its
+// first Stockham pass -- and loops over the R rungs of the chunk, so the
+// spectra are read once per chunk rather than once per rung.  Per rung:
+// Gaussian factor, first pass from registers, remaining passes through LDS,
+// store of the block's pencils in layout `mode`: 0 [k1][q], 1 [q][k1] rows for
+// the row kernel (2-D), 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D
+// axis-3 pass).  The arithmetic is that of k_axis_wave with DIR = +1.
 template <typename T, int LOG2L>
 __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -464,30 +471,32 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   uint32_t bid = blockIdx.x;
   const uint32_t kt = bid % (uint32_t)a.ktiles;
   bid /= (uint32_t)a.ktiles;
-  const uint32_t o = bid & (uint32_t)(a.O - 1);
-  bid >>= a.log2O;
+  const uint32_t ot = bid & ((1u << a.log2otiles) - 1u);
+  bid >>= a.log2otiles;
   const int f = a.fid0 + (int)bid;
-  const int log2TK = a.log2TK;
-  const int TK = 1 << log2TK;
+  const int log2TK = a.log2TK, log2TJ = a.log2TJ;
+  const int TK = 1 << log2TK, TJ = 1 << log2TJ;
   const int k0 = (int)kt * TK;
   const int TKe = (a.H1 - k0) < TK ? (a.H1 - k0) : TK;
-  const int64_t obase = (int64_t)o * L;
+  const int64_t o0 = (int64_t)ot * TJ;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int p = wv * ppw + lane / lp;   // this lane's pencil (column k0 + p)
+  const int p = wv * ppw + lane / lp;   // this lane's pencil: column k0 + kk, outer o0 + jj
+  const int kk = p >> log2TJ, jj = p & (TJ - 1);
   const int t = lane % lp;
-  const bool act = p < TKe;
+  const bool act = kk < TKe && p < TK * TJ;
   const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
   const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
   cplx<T> x[E];
   if (act) {
+    const cplx<T>* col = src + (int64_t)(k0 + kk) * a.Q + (o0 + jj) * L;
 #pragma unroll
     for (int u = 0; u < E / R1; ++u)
 #pragma unroll
-      for (int r = 0; r < R1; ++r)
-        x[u * R1 + r] = src[(int64_t)(k0 + p) * a.Q + obase + t + u * tpp + r * nb1];
+      for (int r = 0; r < R1; ++r) x[u * R1 + r] = col[t + u * tpp + r * nb1];
   }
   cplx<T>* pb = buf + (act ? p : 0) * PL;
-  const int total = TK << LOG2L;
+  const int log2np = log2TK + log2TJ;
+  const int total = 1 << (log2np + LOG2L);
   for (int rr = 0; rr < a.R; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
     if (act) {
@@ -506,17 +515,30 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
     wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
-    if (a.transposed) {
+    if (a.mode == 1) {
+      // [q][k1]: consecutive k1 of one row (TJ == 1)
       for (int id = tid; id < total; id += kThreads) {
-        const int kk = id & (TK - 1);
+        const int k = id & (TK - 1);
         const int e = id >> log2TK;
-        if (kk < TKe) dst[(obase + e) * a.H1p + k0 + kk] = buf[kk * PL + pad8(e)];
+        if (k < TKe) dst[(o0 * L + e) * a.H1p + k0 + k] = buf[k * PL + pad8(e)];
+      }
+    } else if (a.mode == 2) {
+      // [k1][j2][j3]: consecutive j3 (outer index) for one (k1, j2)
+      for (int id = tid; id < total; id += kThreads) {
+        const int j = id & (TJ - 1);
+        const int e = (id >> log2TJ) & (L - 1);
+        const int k = id >> (log2TJ + LOG2L);
+        if (k < TKe)
+          dst[(int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + o0 + j] =
+              buf[((k << log2TJ) + j) * PL + pad8(e)];
       }
     } else {
       for (int id = tid; id < total; id += kThreads) {
         const int e = id & (L - 1);
-        const int kk = id >> LOG2L;
-        if (kk < TKe) dst[(int64_t)(k0 + kk) * a.Q + obase + e] = buf[kk * PL + pad8(e)];
+        const int j = (id >> LOG2L) & (TJ - 1);
+        const int k = id >> (LOG2L + log2TJ);
+        if (k < TKe)
+          dst[(int64_t)(k0 + k) * a.Q + (o0 + j) * L + e] = buf[((k << log2TJ) + j) * PL + pad8(e)];
       }
     }
     __syncthreads();
@@ -553,13 +575,13 @@ static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream
 
 // First inverse pass of a rung chunk along grid axis 2 for the 2B bootstrap
 // fields (fields [0, B) use the means factors, [B, 2B) the variance factors):
-// spec [field][k1][q] -> dst rungs [r0, r0 + R).  Returns false when the axis
-// length is outside the wave-local range (the caller then uses the per-rung
-// pass).
+// spec [field][k1][q] -> dst rungs [r0, r0 + R) in layout `mode` (see
+// k_axis_ladder).  Returns false when the axis length is outside the
+// wave-local range (the caller then uses the per-rung pass).
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
-                        const T* gV, int fsplit, const T* tw, int transposed, hipStream_t s) {
+                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s) {
   AxisArgs<T> a{};
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[1];
@@ -570,7 +592,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.src_rs = 0;
   a.dst_rs = dst_rung_stride;
   a.fsize = (int64_t)g.H1 * g.Q;
-  a.dst_fsize = transposed ? (int64_t)g.Q * g.H1p : a.fsize;
+  a.dst_fsize = mode == 1 ? (int64_t)g.Q * g.H1p : a.fsize;
   a.fid0 = 0;
   a.nf = nf;
   a.src_fid0 = 0;
@@ -588,6 +610,72 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.I = 1;
   a.O = g.n[2];
   a.TI = 1;
+  a.mode = mode;
+  const int per_block = (kThreads * Ef) / a.L;
+  int hp = 1;
+  while (hp < g.H1) hp <<= 1;
+  a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(8, per_block)) : 1;
+  a.TK = std::min(hp, per_block / a.TJ);
+  a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+  a.itiles = 1;
+  a.log2TI = 0;
+  a.log2TK = ilog2c(a.TK);
+  a.log2TJ = ilog2c(a.TJ);
+  a.log2otiles = ilog2c((int)(a.O / a.TJ));
+  a.log2itiles = 0;
+  a.log2O = ilog2c((int)a.O);
+  a.transposed = mode == 1;
+  a.tw = (const cplx<T>*)tw;
+  const int64_t blocks = (int64_t)nf * (a.O / a.TJ) * a.ktiles;
+  const size_t lds = (size_t)a.TK * a.TJ * pencil_stride(a.L) * sizeof(cplx<T>);
+  return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
+}
+
+template <typename T, int DIR>
+static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
+                             const AxisArgs<T>& a);
+
+// Second inverse pass of a 3-D rung chunk (grid axis 3) on the [k1][j2][j3]
+// layout written by launch_axis_ladder (mode 2): pencils along j3 are
+// contiguous; the output is the transposed [q][k1] layout of the row kernel
+// (q = j2 + n2 * j3).  False when n3 is outside the wave-local range.
+template <typename T>
+bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride,
+                       cplx<T>* dst, int64_t dst_rung_stride, int nf, int R, int r0,
+                       const T* gM, const T* gV, int fsplit, const T* tw, hipStream_t s) {
+  AxisArgs<T> a{};
+  const int Ef = sizeof(T) == 8 ? 8 : 16;
+  a.L = g.n[2];
+  a.log2L = g.log2n[2];
+  if (g.N != 3 || a.L < Ef || a.L > 64 * Ef) return false;
+  a.src = src;
+  a.dst = dst;
+  a.src_rs = src_rung_stride;
+  a.dst_rs = dst_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.dst_fsize = (int64_t)g.Q * g.H1p;
+  a.fid0 = 0;
+  a.nf = nf;
+  a.src_fid0 = 0;
+  a.dst_fid0 = 0;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.fsplit = fsplit;
+  a.ksum = g.ksum;
+  a.goff = g.koff[2];
+  a.Q = g.Q;
+  a.H1 = g.H1;
+  a.H1p = g.H1p;
+  // input: pencil (k1, j2) at k1 * Q + j2 * n3, contiguous along j3
+  a.I = 1;
+  a.O = g.n[1];
+  a.TI = 1;
+  a.transposed = 1;
+  a.oO = 1;          // q_out = j2 + n2 * j3
+  a.oE = g.n[1];
+  a.oI = 0;
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
   while (hp < g.H1) hp <<= 1;
@@ -598,12 +686,18 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.log2TK = ilog2c(a.TK);
   a.log2itiles = 0;
   a.log2O = ilog2c((int)a.O);
-  a.transposed = transposed;
   a.tw = (const cplx<T>*)tw;
-  const int64_t blocks = (int64_t)nf * a.O * a.ktiles;
+  const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles;
   const size_t lds = (size_t)a.TK * pencil_stride(a.L) * sizeof(cplx<T>);
-  return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
+  axis_wave_launch<T, 1>(a.log2L, (unsigned)blocks, lds, s, a);
+  return true;
 }
+template bool launch_axis3_rows<double>(const GridDev&, const cplx<double>*, int64_t,
+                                        cplx<double>*, int64_t, int, int, int, const double*,
+                                        const double*, int, const double*, hipStream_t);
+template bool launch_axis3_rows<float>(const GridDev&, const cplx<float>*, int64_t, cplx<float>*,
+                                       int64_t, int, int, int, const float*, const float*, int,
+                                       const float*, hipStream_t);
 template bool launch_axis_ladder<double>(const GridDev&, const cplx<double>*, cplx<double>*,
                                          int64_t, int, int, int, const double*, const double*,
                                          int, const double*, int, hipStream_t);
@@ -680,6 +774,9 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
   a.transposed = transposed;
   a.tw = (const cplx<TI>*)tw;
+  a.oO = (int64_t)a.L * a.I;  // q_out = q_in
+  a.oE = a.I;
+  a.oI = 1;
   if (a.L >= Ef && a.L <= 64 * Ef) {
     // wave-local form: TI and TK powers of two, TK * TI <= per_block
     a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, std::min(8, per_block));

@@ -59,12 +59,20 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
                      int transposed, hipStream_t s);
 
 // Ladder form of the first inverse pass along grid axis 2: spectra read once
-// per chunk of R rungs (see k_axis_ladder).  False if the axis length is
-// outside the wave-local range.
+// per chunk of R rungs (see k_axis_ladder).  mode: 0 [k1][q], 1 [q][k1]
+// (row-kernel layout), 2 [k1][j2][j3] (input of launch_axis3_rows).  False if
+// the axis length is outside the wave-local range.
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
-                        const T* gV, int fsplit, const T* tw, int transposed, hipStream_t s);
+                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s);
+
+// 3-D: inverse pass along grid axis 3 from the [k1][j2][j3] layout to the
+// row-kernel layout [q][k1].  False if n3 is outside the wave-local range.
+template <typename T>
+bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride,
+                       cplx<T>* dst, int64_t dst_rung_stride, int nf, int R, int r0,
+                       const T* gM, const T* gV, int fsplit, const T* tw, hipStream_t s);
 
 template <typename TI, typename TO>
 void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream_t s);

@@ -27,6 +27,8 @@ struct pkde_ctx {
   float* tw32 = nullptr;
   std::mutex mu;
   uint64_t chunk_bytes = 4ull << 30;  // per-plan budget of the rung-chunk buffer
+  bool chunk_fixed = false;           // budget set by PKDE_CHUNK_BYTES
+  uint64_t mem_total = 0;             // device memory, bytes
   // batched form
   std::vector<pkde_batch_lane> lanes;
   unsigned char* stage = nullptr;  // pinned host staging
