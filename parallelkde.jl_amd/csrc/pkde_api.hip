@@ -77,7 +77,7 @@ static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
 // memory (the first axis pass reads the spectra once per chunk).
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
-  const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.Q * pl->g.H1p * 2 * rs;
+  const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
   const uint64_t units = (uint64_t)y_units(pl);
   uint64_t rungs = std::max<uint64_t>(1, pl->ctx->chunk_bytes / units / per_rung);
   if (!pl->ctx->chunk_fixed)
@@ -222,7 +222,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   }
   for (int ax = 1; ax < N; ++ax)
     launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
-                                        nullptr, nullptr, 0, pl->ctx->tw64, 0, s);
+                                        nullptr, nullptr, 0, pl->ctx->tw64, s);
   const cplx<T>* spec;
   if (sizeof(T) == 4) {
     launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
@@ -240,11 +240,14 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   } else {
     // axis 2 (and 3) inverse passes write the transposed layout [q][k1] the
     // row kernel reads; in 3-D the axis-2 result goes through a second buffer
-    const int64_t per_rung = 2 * (int64_t)B * g.Q * g.H1p;  // padded rows
-    const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512;
+    const int64_t per_rung = 2 * (int64_t)B * g.fsY;  // tiled rows (>= H1 * Q per field)
     const int Ew = sizeof(T) == 8 ? 8 : 16;  // wave-local FFT range [E, 64 E]
-    const bool rows3 = N == 3 && rows && g.n[1] >= Ew && g.n[1] <= 64 * Ew && g.n[2] >= Ew &&
-                       g.n[2] <= 64 * Ew;
+    auto wave_ok = [&](int L) { return L >= Ew && L <= 64 * Ew; };
+    // the row kernel needs the axes before it in the wave-local range (their
+    // passes write the tiled row layout)
+    const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
+                      (N == 2 || wave_ok(g.n[2]));
+    const bool rows3 = N == 3 && rows;
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
     const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
@@ -257,15 +260,18 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       // so the axis-3 pass reads whole pencils and writes [q][k1] rows.
       cplx<T>* Y1 = N == 2 ? Y : Y2;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
-      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s))
+      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
+        if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
-                                 mode1 == 1 ? 1 : 0, s);
+                                 s);
+      }
       if (N == 3) {
         if (!rows3)
           launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV,
-                                   B, tw, rows ? 1 : 0, s);
-        else
-          launch_axis3_rows<T>(g, Y2, per_rung, Y, per_rung, 2 * B, R, r0, gM, gV, B, tw, s);
+                                   B, tw, s);
+        else if (!launch_axis3_rows<T>(g, Y2, per_rung, Y, per_rung, 2 * B, R, r0, gM, gV, B, tw,
+                                       s))
+          return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
       }
       if (rows)
         launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
@@ -289,9 +295,9 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const int fM = 2 * B;
     for (int r0 = 0; r0 < S; r0 += Rm) {
       const int R = std::min(Rm, S - r0);
-      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, 0, s);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
       if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, 0, s);
+        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
       launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
     }
   }
@@ -455,10 +461,21 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     g.G *= g.n[i];
   }
   g.H1 = g.n[0] / 2 + 1;
-  g.H1p = (g.H1 + 7) / 8 * 8;
   g.Q = g.G / g.n[0];
   pl->B = p.n_bootstraps;
   pl->prec = p.precision ? 1 : 0;
+  {
+    // tiled row layout: tile width = the k1 columns one block of the axis pass
+    // writing it holds (grid axis 2 in 2-D, axis 3 in 3-D; 256 lanes x E values)
+    const int E = pl->prec ? 8 : 16;
+    const int L = ndims == 3 ? g.n[2] : g.n[1];
+    int hp = 1;
+    while (hp < g.H1) hp <<= 1;
+    g.TKy = ndims >= 2 ? std::min(hp, std::max(1, 256 * E / L)) : 1;
+    g.log2TKy = ilog2(g.TKy);
+    g.ktY = (g.H1 + g.TKy - 1) / g.TKy;
+    g.fsY = (int64_t)g.ktY * g.Q * g.TKy;
+  }
   pl->fixed = p.deterministic_binning ? 1 : 0;
   // defaults (GradeProEstimator.jl:541-550, 754-766)
   pl->alpha_s = isnan(p.alpha_s) ? (ndims == 1 ? 0.0 : 0.06) : p.alpha_s;

@@ -274,7 +274,7 @@ struct AxisArgs {
   cplx<T>* dst;
   int64_t src_rs, dst_rs;  // rung strides (elements)
   int64_t fsize;           // H1 * Q
-  int64_t dst_fsize;       // H1 * Q, or Q * H1p when transposed
+  int64_t dst_fsize;       // H1 * Q, or fsY for the tiled row layout
   int fid0, nf, src_fid0, dst_fid0;
   int R, r0;
   const T* gM;
@@ -282,12 +282,12 @@ struct AxisArgs {
   int fsplit, ksum, goff;
   int L, log2L;
   int64_t I, O, Q;
-  int H1, H1p, TK, TI, ktiles, itiles;
+  int H1, TK, TI, ktiles, itiles;
   int log2TK, log2TI, log2itiles, log2O;
-  int transposed;
-  // output pencil placement of the transposed store: q_out = o * oO + e * oE + i * oI
-  // (o: outer pencil index, e: position along the axis, i: inner index)
-  int64_t oO, oE, oI;
+  int transposed;  // store the tiled row layout (TK == GridDev::TKy)
+  // row index of the tiled store: q' = o * oO + e * oE
+  // (o: outer pencil index, e: position along the axis)
+  int64_t oO, oE;
   // ladder pass: TJ outer pencils per block, store mode 0 [k1][q], 1 [q][k1],
   // 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass)
   int TJ, log2TJ, log2otiles, mode;
@@ -342,11 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
   lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
   for (int id = tid; id < total; id += nthr) {
     int kk, ii, e;
-    if (a.transposed) {
-      kk = id % TKe;
-      ii = (id / TKe) % TI;
-      e = id / (TKe * TI);
-    } else if (TI == 1) {
+    if (TI == 1) {
       kk = id / L;
       e = id - kk * L;
       ii = 0;
@@ -356,11 +352,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
       kk = id / (TI * L);
     }
     const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
-    const cplx<T> v = buf[(kk * TI + ii) * L + e];
-    if (a.transposed)
-      dst[q * a.H1p + k0 + kk] = v;
-    else
-      dst[(int64_t)(k0 + kk) * a.Q + q] = v;
+    dst[(int64_t)(k0 + kk) * a.Q + q] = buf[(kk * TI + ii) * L + e];
   }
 }
 
@@ -423,13 +415,14 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   }
   __syncthreads();
   if (a.transposed) {
+    // tiled rows (TI == 1): this block's tile is kt, its rows o * oO + e * oE
+    cplx<T>* tile = dst + (int64_t)kt * a.Q * TK;
     for (int id = tid; id < total; id += kThreads) {
       const int kk = id & (TK - 1);
-      const int ii = (id >> log2TK) & (TI - 1);
-      const int e = id >> (log2TK + log2TI);
+      const int e = id >> log2TK;
       if (kk < TKe) {
-        const int64_t q = (int64_t)o * a.oO + (int64_t)e * a.oE + (i0 + ii) * a.oI;
-        dst[q * a.H1p + k0 + kk] = buf[(kk * TI + ii) * PL + pad8(e)];
+        const int64_t q = (int64_t)o * a.oO + (int64_t)e * a.oE;
+        tile[q * TK + kk] = buf[kk * PL + pad8(e)];
       }
     }
   } else {
@@ -516,11 +509,12 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if (a.mode == 1) {
-      // [q][k1]: consecutive k1 of one row (TJ == 1)
+      // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous
+      cplx<T>* tile = dst + (int64_t)kt * a.Q * TK;
       for (int id = tid; id < total; id += kThreads) {
         const int k = id & (TK - 1);
         const int e = id >> log2TK;
-        if (k < TKe) dst[(o0 * L + e) * a.H1p + k0 + k] = buf[k * PL + pad8(e)];
+        if (k < TKe) tile[(o0 * L + e) * TK + k] = buf[k * PL + pad8(e)];
       }
     } else if (a.mode == 2) {
       // [k1][j2][j3]: consecutive j3 (outer index) for one (k1, j2)
@@ -592,7 +586,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.src_rs = 0;
   a.dst_rs = dst_rung_stride;
   a.fsize = (int64_t)g.H1 * g.Q;
-  a.dst_fsize = mode == 1 ? (int64_t)g.Q * g.H1p : a.fsize;
+  a.dst_fsize = mode == 1 ? g.fsY : a.fsize;
   a.fid0 = 0;
   a.nf = nf;
   a.src_fid0 = 0;
@@ -606,7 +600,6 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.goff = g.koff[1];
   a.Q = g.Q;
   a.H1 = g.H1;
-  a.H1p = g.H1p;
   a.I = 1;
   a.O = g.n[2];
   a.TI = 1;
@@ -616,6 +609,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   while (hp < g.H1) hp <<= 1;
   a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(8, per_block)) : 1;
   a.TK = std::min(hp, per_block / a.TJ);
+  if (mode == 1 && a.TK != g.TKy) return false;  // tile width fixed by the layout
   a.ktiles = (g.H1 + a.TK - 1) / a.TK;
   a.itiles = 1;
   a.log2TI = 0;
@@ -653,7 +647,7 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.src_rs = src_rung_stride;
   a.dst_rs = dst_rung_stride;
   a.fsize = (int64_t)g.H1 * g.Q;
-  a.dst_fsize = (int64_t)g.Q * g.H1p;
+  a.dst_fsize = g.fsY;
   a.fid0 = 0;
   a.nf = nf;
   a.src_fid0 = 0;
@@ -667,19 +661,18 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.goff = g.koff[2];
   a.Q = g.Q;
   a.H1 = g.H1;
-  a.H1p = g.H1p;
   // input: pencil (k1, j2) at k1 * Q + j2 * n3, contiguous along j3
   a.I = 1;
   a.O = g.n[1];
   a.TI = 1;
   a.transposed = 1;
-  a.oO = 1;          // q_out = j2 + n2 * j3
-  a.oE = g.n[1];
-  a.oI = 0;
+  a.oO = g.n[2];     // row q' = j3 + n3 * j2: the block's rows are contiguous
+  a.oE = 1;
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
   while (hp < g.H1) hp <<= 1;
   a.TK = std::min(hp, per_block);
+  if (a.TK != g.TKy) return false;  // tile width fixed by the layout
   a.ktiles = (g.H1 + a.TK - 1) / a.TK;
   a.itiles = 1;
   a.log2TI = 0;
@@ -737,7 +730,7 @@ template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     int transposed, hipStream_t s) {
+                     hipStream_t s) {
   static_assert(sizeof(TI) == sizeof(TO), "axis passes keep the element type");
   AxisArgs<TI> a;
   a.src = src;
@@ -760,8 +753,7 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.log2L = g.log2n[axis];
   a.Q = g.Q;
   a.H1 = g.H1;
-  a.H1p = g.H1p;
-  a.dst_fsize = transposed ? (int64_t)g.Q * g.H1p : a.fsize;
+  a.dst_fsize = a.fsize;
   if (axis == 1) {  // grid axis 2: contiguous, outer index j3
     a.I = 1;
     a.O = g.n[2];
@@ -772,11 +764,8 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   // pencils per workgroup: one per L/E lanes (E = 8 in FP64, 16 in FP32)
   const int Ef = sizeof(TI) == 8 ? 8 : 16;
   const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
-  a.transposed = transposed;
+  a.transposed = 0;
   a.tw = (const cplx<TI>*)tw;
-  a.oO = (int64_t)a.L * a.I;  // q_out = q_in
-  a.oE = a.I;
-  a.oI = 1;
   if (a.L >= Ef && a.L <= 64 * Ef) {
     // wave-local form: TI and TK powers of two, TK * TI <= per_block
     a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, std::min(8, per_block));
@@ -810,8 +799,7 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
 #define PKDE_AXIS_INST(TI, TO, DIR)                                                            \
   template void launch_axis_fft<TI, TO, DIR>(const GridDev&, int, const cplx<TI>*, int64_t, int,  \
                                              cplx<TO>*, int64_t, int, int, int, int, int,        \
-                                             const TI*, const TI*, int, const TI*, int,          \
-                                             hipStream_t);
+                                             const TI*, const TI*, int, const TI*, hipStream_t);
 PKDE_AXIS_INST(double, double, -1)
 PKDE_AXIS_INST(double, double, 1)
 PKDE_AXIS_INST(float, float, 1)
@@ -878,7 +866,10 @@ struct VmrArgs {
   double* vmr;
   int64_t G, Q;
   int n1, log2n1, H1, TQ, NBP, ntiles;
-  int64_t rowp;  // row stride of the transposed layout (rows kernel)
+  // tiled row layout (rows kernel): tile width 2^log2TKy, tile stride tstride;
+  // rows3: rows are q' = j3 + n3 * j2 (3-D), grid row q = j2 + n2 * j3
+  int log2TKy, rows3, log2n2, log2n3;
+  int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
 };
 
@@ -1061,8 +1052,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   const int r = a.r0 + rr;
   const T* gm1 = a.gM + (int64_t)r * a.ksum;
   const T* gv1 = a.gV + (int64_t)r * a.ksum;
-  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + q0 * a.rowp;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs;
   const double vfac = a.cV[r] * a.invG;
+  // tiled rows: (row, k1) at (k1 >> log2TKy) * tstride + row * TKy + (k1 & (TKy - 1))
+  const int ltk = a.log2TKy;
+  const int tkm = (1 << ltk) - 1;
+  auto toff = [&](int row, int k1) -> int64_t {
+    return (int64_t)(k1 >> ltk) * a.tstride + ((q0 + row) << ltk) + (k1 & tkm);
+  };
 
   // Per lane: PPT (row, k1) items of the half spectrum k1 < n1/2, plus the
   // Nyquist term k1 = n1/2 of row `lane` for lanes < 2W.  Row index
@@ -1078,7 +1075,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   const int nrow = nyq ? lane : 0;
   const int nty = nrow >> LOG2W, nw = nrow & (W - 1);
   const T ng = nty ? gv1[H] : gm1[H];
-  const int nyq_off = nw * (int)a.rowp + H;
+  const int64_t nyq_off = toff(nw, H);
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -1097,7 +1094,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       const int id = lane + 64 * u;
       const int k1 = id & (H - 1);
       const int rw = id >> (LOG2N1 - 1);
-      const int off = (rw & (W - 1)) * (int)a.rowp + k1;
+      const int64_t off = toff(rw & (W - 1), k1);
       const cplx<T>* fb = (rw >> LOG2W) ? fS : f0;
       const cplx<T> A = cscale(fb[off], ld_g[u]);
       cplx<T> Bv = {(T)0, (T)0};
@@ -1152,8 +1149,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   for (int u = 0; u < PPT; ++u) {
     const int pt = lane + 64 * u;
     const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
+    const int64_t qr = q0 + w;  // row of the layout -> grid row
+    const int64_t qg = a.rows3 ? (qr >> a.log2n3) + ((qr & ((1 << a.log2n3) - 1)) << a.log2n2) : qr;
     const double vv = sc * m2[u] / (double)(B - 1);
-    a.vmr[(int64_t)r * a.G + (q0 + w) * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+    a.vmr[(int64_t)r * a.G + qg * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
   }
 }
 
@@ -1194,8 +1193,12 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   const int W = 64 * ppt / n1;
   a.src = src;
   a.src_rs = src_rung_stride;
-  a.fsize = (int64_t)g.Q * g.H1p;
-  a.rowp = g.H1p;
+  a.fsize = g.fsY;
+  a.log2TKy = g.log2TKy;
+  a.tstride = g.Q << g.log2TKy;
+  a.rows3 = g.N == 3;
+  a.log2n2 = g.log2n[1];
+  a.log2n3 = g.log2n[2];
   a.B = B;
   a.invn = 1.0 / (double)n_samples;
   a.invG = 1.0 / (double)g.G;

@@ -18,7 +18,12 @@ struct GridDev {
   int log2n[kMaxDims];
   int64_t G;   // prod n
   int H1;      // n1/2 + 1 Hermitian half along axis 1
-  int H1p;     // H1 rounded up to 8: row stride of the transposed [q][k1] layout
+  // Row-kernel layout of a rung chunk ("tiled rows"): element (field, row
+  // q', k1) at field * fsY + (k1 / TKy) * (Q * TKy) + q' * TKy + k1 % TKy, so
+  // the axis pass that writes it stores each block's TKy columns as one
+  // contiguous tile.  Rows are q' = q in 2-D and q' = j3 + n3 * j2 in 3-D.
+  int TKy, log2TKy, ktY;
+  int64_t fsY;
   int64_t Q;   // n2 * n3 (pencils per axis-1 row set)
   double lo[kMaxDims], hi[kMaxDims], h[kMaxDims];
   int koff[kMaxDims];  // offset of axis i in per-rung Gaussian tables
@@ -51,16 +56,17 @@ void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double s
                           hipStream_t s);
 
 // Generic batched pencil FFT along axis `axis` (>= 1, i.e. axis 2 or 3) of
-// [rung][field][k1][q] arrays.  Optional per-rung Gaussian factors.
+// [rung][field][k1][q] arrays, in place or out of place, same layout.
+// Optional per-rung Gaussian factors.
 template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     int transposed, hipStream_t s);
+                     hipStream_t s);
 
 // Ladder form of the first inverse pass along grid axis 2: spectra read once
-// per chunk of R rungs (see k_axis_ladder).  mode: 0 [k1][q], 1 [q][k1]
-// (row-kernel layout), 2 [k1][j2][j3] (input of launch_axis3_rows).  False if
+// per chunk of R rungs (see k_axis_ladder).  mode: 0 [k1][q], 1 tiled rows
+// (row-kernel layout, 2-D), 2 [k1][j2][j3] (input of launch_axis3_rows).  False if
 // the axis length is outside the wave-local range.
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
@@ -68,7 +74,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         const T* gV, int fsplit, const T* tw, int mode, hipStream_t s);
 
 // 3-D: inverse pass along grid axis 3 from the [k1][j2][j3] layout to the
-// row-kernel layout [q][k1].  False if n3 is outside the wave-local range.
+// tiled row-kernel layout.  False if n3 is outside the wave-local range.
 template <typename T>
 bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride,
                        cplx<T>* dst, int64_t dst_rung_stride, int nf, int R, int r0,
@@ -86,7 +92,7 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
                        hipStream_t s);
 
 // Row-per-wave form of the same ladder step for Q >= 64 rows and
-// 16 <= n1 <= 512, reading the transposed layout [rung][field][q][k1].
+// 16 <= n1 <= 512, reading the tiled row layout (see GridDev).
 template <typename T>
 void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
