@@ -106,6 +106,19 @@ void pkde_gradepro_plan_destroy(pkde_plan *plan);
  * GradeProEstimator.jl:1091-1096). */
 uint64_t pkde_gradepro_plan_footprint(const pkde_plan *plan, int64_t n_samples);
 
+/* Rule-of-thumb plan: the :rot estimator (RoTEstimator.jl:116-265 --
+ * initialize_estimator, estimate!): full-sample binning, forward FFT, one
+ * Gaussian propagation by t_i = factor * sigma_i - h_i / 2, normalised
+ * inverse FFT, |.|.  rule: 0 Scott (the reference's default,
+ * RoTEstimator.jl:120), 1 Silverman; precision as in pkde_gradepro_params.
+ * Estimate with pkde_gradepro_estimate / _estimate_device (bootstrap_idx is
+ * ignored; the host form writes chosen rung 0; the device form does not
+ * write chosen); pkde_gradepro_last_stats reports the propagation time per
+ * axis in bw_final. */
+int pkde_rot_plan_create(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis, const double *lo,
+                         const double *hi, const double *h, int rule, int precision,
+                         pkde_plan **out);
+
 /* The plain form: estimate!(::GradeProEstimator, kde) (GradeProEstimator.jl:
  * 968-1051) + replace_nans! (:1053-1072).  Synchronous; copies the host
  * samples in, writes caller-owned host outputs.  bootstrap_idx may be NULL

@@ -728,3 +728,70 @@ done:
   if (!chosen) free(ch);
   return rc;
 }
+
+/* RoTEstimator.jl:116-265, :serial. */
+int og_rot_estimate(const og_grid *g, const double *samples, int64_t n, int rule,
+                    double *density, double *time_out) {
+  const int N = g->ndims;
+  if (N < 1 || N > OG_MAX_DIMS || n < 2 || (rule != 0 && rule != 1)) return OG_INVALID_ARGUMENT;
+  int rc = og_check_data(g, samples, n); /* initialize_dirac_sequence -> check_data */
+  if (rc != OG_OK) return rc;
+  const int64_t G = og_grid_points(g);
+  /* propagation_time: factor .* sqrt.(diag(cov(data, dims=2))) (n - 1 normalised) */
+  const double ex = -1.0 / (double)(N + 4);
+  const double factor = rule == 1 ? pow((double)(n * (int64_t)(N + 2)) / 4.0, ex)
+                                  : pow((double)n, ex);
+  double t[OG_MAX_DIMS], t0[OG_MAX_DIMS];
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+    for (int64_t k = 0; k < n; ++k) s += samples[k * N + i];
+    const double mean = s / (double)n;
+    double ss = 0.0;
+    for (int64_t k = 0; k < n; ++k) {
+      const double d = samples[k * N + i] - mean;
+      ss += d * d;
+    }
+    t0[i] = g->h[i] / 2.0; /* initial_bandwidth, Grids.jl:217 */
+    t[i] = factor * sqrt(ss / (double)(n - 1)) - t0[i];
+    if (time_out) time_out[i] = t[i];
+  }
+  int32_t *idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  double *re = (double *)malloc(sizeof(double) * (size_t)G);
+  double *f0 = (double *)malloc(sizeof(double) * 2 * (size_t)G);
+  double *ft = (double *)malloc(sizeof(double) * 2 * (size_t)G);
+  int64_t ksum = 0;
+  for (int i = 0; i < N; ++i) ksum += g->n[i];
+  double *kgrid = (double *)malloc(sizeof(double) * (size_t)ksum);
+  rc = OG_OUT_OF_MEMORY;
+  if (!idx || !re || !f0 || !ft || !kgrid) goto done;
+  for (int64_t k = 0; k < n; ++k) idx[k] = (int32_t)k; /* no bootstraps: identity */
+  rc = og_dirac_sequences(g, samples, n, idx, 1, re, NULL, 1);
+  if (rc != OG_OK) goto done;
+  for (int64_t j = 0; j < G; ++j) {
+    f0[2 * j] = re[j];
+    f0[2 * j + 1] = 0.0;
+  }
+  og_fft_nd(N, g->n, f0, -1);
+  {
+    int64_t off = 0;
+    for (int i = 0; i < N; ++i) {
+      og_fftfreq(g->n[i], g->h[i], kgrid + off);
+      off += g->n[i];
+    }
+  }
+  og_propagate(g, kgrid, t, t0, f0, NULL, ft, NULL);
+  og_fft_nd(N, g->n, ft, +1);
+  {
+    const double ginv = 1.0 / (double)G; /* plan_ifft! normalisation */
+    for (int64_t j = 0; j < 2 * G; ++j) ft[j] *= ginv;
+  }
+  og_full_means(G, ft, density); /* copy_density!: abs */
+  rc = OG_OK;
+done:
+  free(idx);
+  free(re);
+  free(f0);
+  free(ft);
+  free(kgrid);
+  return rc;
+}

@@ -135,6 +135,17 @@ int og_gradepro_estimate(const og_grid *g, const double *samples, int64_t n,
                          double *density, int16_t *chosen, double *vmr_table,
                          double *means_table, int32_t max_rungs, og_info *info);
 
+/* ---- the rule-of-thumb estimator ----
+ * estimate_density!(est, :rot) on a grid, :serial (RoTEstimator.jl:116-265):
+ * full-sample Dirac sequence and forward FFT (initialize_density, :191-202),
+ * propagation of the means by t_i = factor * sigma_i - h_i / 2
+ * (propagation_time :22-28 Silverman / :42-47 Scott, RoTEstimator :67-70,
+ * propagate_kernels! :219-235), normalised inverse FFT and |.|
+ * (ifft_density!, copy_density! :237-265).  rule: 0 Scott, 1 Silverman.
+ * time_out (may be NULL) receives t. */
+int og_rot_estimate(const og_grid *g, const double *samples, int64_t n, int rule,
+                    double *density, double *time_out);
+
 #ifdef __cplusplus
 }
 #endif

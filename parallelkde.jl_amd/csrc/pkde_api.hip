@@ -79,6 +79,7 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
   const uint64_t units = (uint64_t)y_units(pl);
+  if (per_rung == 0) return (uint64_t)fsize(pl) * 2 * rs * 2;  // :rot: the means pass only
   uint64_t rungs = std::max<uint64_t>(1, pl->ctx->chunk_bytes / units / per_rung);
   if (!pl->ctx->chunk_fixed)
     rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
@@ -183,6 +184,35 @@ static void plan_free(pkde_plan* pl) {
     if (e) (void)hipEventDestroy(e);
 }
 
+// Full-sample means at each point's selected rung (d.sel) -> density: the
+// means field's inverse passes for the rungs in use, then |.| at the selected
+// rung (calculate_full_means!, DirectSpace.jl:747-764; replace_nans!).
+template <typename T>
+static int enqueue_means(pkde_plan* pl, const cplx<T>* spec, const T* gM, const T* tw,
+                         double* d_density, hipStream_t s) {
+  PlanBuffers& d = pl->d;
+  const GridDev& g = pl->g;
+  const int B = pl->B, S = pl->S, N = g.N;
+  const int64_t fs = fsize(pl);
+  if (N == 1) {
+    launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, d.sel, d_density, s);
+  } else {
+    if (d.Y_elems < 2 * fs) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than two rungs");
+    int Rm = (int)std::min<int64_t>(S, d.Y_elems / fs);
+    if (Rm > 1 && (Rm & 1)) --Rm;
+    cplx<T>* Y = (cplx<T>*)d.Y;
+    const int fM = 2 * B;
+    for (int r0 = 0; r0 < S; r0 += Rm) {
+      const int R = std::min(Rm, S - r0);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      if (N == 3)
+        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
+    }
+  }
+  return PKDE_OK;
+}
+
 template <typename T>
 static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
                          double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
@@ -285,22 +315,64 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   // ---- convergence + means at the chosen rung ----
   int16_t* chosen = d_chosen ? d_chosen : d.chosen;
   launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, s);
-  if (N == 1) {
-    launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, d.sel, d_density, s);
+  const int rc = enqueue_means<T>(pl, spec, gM, tw, d_density, s);
+  if (rc) return rc;
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
+  return check_hip(hipGetLastError(), "kernel launch");
+}
+
+// :rot (RoTEstimator.jl:147-265): full-sample binning and forward FFT
+// (initialize_density, :191-202), one Gaussian propagation by the rule of
+// thumb minus h/2 (propagate_kernels!, :219-235), normalised inverse FFT and
+// |.| (ifft_density! + copy_density!, :237-265).  The means pass of the
+// :gradepro path does the last two steps with every point at rung 0.
+template <typename T>
+static int enqueue_rot(pkde_plan* pl, const double* d_x, int64_t n, double* d_density,
+                       hipStream_t s, bool timed) {
+  PlanBuffers& d = pl->d;
+  const GridDev& g = pl->g;
+  const int N = g.N;
+  const int64_t G = g.G, fs = fsize(pl);
+  const T* tw = (const T*)(sizeof(T) == 8 ? (const void*)pl->ctx->tw64 : (const void*)pl->ctx->tw32);
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
+  launch_fill_i32(d.err, 0, 1, s);
+  const bool sorted = pl->fixed && d.binwork;
+  if (!sorted) launch_zero_bytes(d.acc, G * 8, s);
+  // sample standard deviations (prefactor 1) -> propagation time -> factors
+  launch_silverman(N, d_x, n, 1.0, d.partial, d.mean, d.bw, s);
+  launch_rot_times(g, n, pl->rot_rule, d.bw, d.times, s);
+  launch_gauss_tables<T>(g, 1, d.times, d.kfreq, (T*)d.gM, (T*)d.gV, s);
+  if (sorted) {
+    if (launch_binning_sorted(g, d_x, n, nullptr, 0, 0, frac_bits_for(n), d.acc, d.err, d.binwork,
+                              d.binwork_bytes, s))
+      return fail(PKDE_DEVICE_ERROR, "binning workspace does not fit the estimate");
   } else {
-    if (d.Y_elems < 2 * fs) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than two rungs");
-    int Rm = (int)std::min<int64_t>(S, d.Y_elems / fs);
-    if (Rm > 1 && (Rm & 1)) --Rm;
-    cplx<T>* Y = (cplx<T>*)d.Y;
-    const int fM = 2 * B;
-    for (int r0 = 0; r0 < S; r0 += Rm) {
-      const int R = std::min(Rm, S - r0);
-      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
-      if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
-      launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
-    }
+    launch_binning(g, d_x, n, nullptr, 0, 0, pl->fixed, frac_bits_for(n), d.acc, d.err, s);
   }
+  {
+    double hp = g.h[0];
+    for (int i = 1; i < N; ++i) hp *= g.h[i];
+    const double two_fb = ldexp(1.0, frac_bits_for(n));
+    const double scaleM = (double)n * hp * two_fb;
+    const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
+    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, 0, d.specD, pl->ctx->tw64, s);
+  }
+  for (int ax = 1; ax < N; ++ax)
+    launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, 1, 1, 0, nullptr,
+                                        nullptr, 0, pl->ctx->tw64, s);
+  const cplx<T>* spec;
+  if (sizeof(T) == 4) {
+    launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, fs, s);
+    spec = (const cplx<T>*)d.spec;
+  } else {
+    spec = (const cplx<T>*)d.specD;
+  }
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[1], s), "event");
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
+  launch_zero_bytes(d.sel, G * 2, s);     // every point at the one rung
+  launch_zero_bytes(d.chosen, G * 2, s);
+  const int rc = enqueue_means<T>(pl, spec, (const T*)d.gM, tw, d_density, s);
+  if (rc) return rc;
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
   return check_hip(hipGetLastError(), "kernel launch");
 }
@@ -312,6 +384,9 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
   int rc = plan_reserve(pl, n, false);
   if (rc) return rc;
   pl->last_n = n;
+  if (pl->kind == 1)
+    return pl->prec ? enqueue_rot<double>(pl, d_x, n, d_density, s, timed)
+                    : enqueue_rot<float>(pl, d_x, n, d_density, s, timed);
   if (pl->prec) return enqueue_typed<double>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
   return enqueue_typed<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
 }
@@ -550,6 +625,25 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
   return PKDE_OK;
 }
 
+int pkde_rot_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
+                         const double* hi, const double* h, int rule, int precision,
+                         pkde_plan** out) {
+  if (rule != 0 && rule != 1) return fail(PKDE_INVALID_ARGUMENT, "Rule of thumb not implemented.");
+  pkde_gradepro_params p;
+  pkde_gradepro_params_default(&p);
+  p.n_steps = 1;
+  p.precision = precision ? 1 : 0;
+  const int rc = pkde_gradepro_plan_create(ctx, ndims, n_per_axis, lo, hi, h, &p, out);
+  if (rc) return rc;
+  pkde_plan* pl = *out;
+  pl->kind = 1;
+  pl->rot_rule = rule;
+  pl->B = 0;
+  pl->S = 1;
+  pl->bw_given = false;
+  return PKDE_OK;
+}
+
 void pkde_gradepro_plan_destroy(pkde_plan* plan) {
   if (!plan) return;
   (void)hipSetDevice(plan->ctx->device);
@@ -568,7 +662,10 @@ static int finish_stats(pkde_plan* pl, int64_t n, pkde_stats* st) {
   st->n_rungs = pl->S;
   double par[3] = {0, 0, 0};
   PKDE_TRY(hipMemcpy(par, pl->d.params, sizeof(par), hipMemcpyDeviceToHost), "params");
-  PKDE_TRY(hipMemcpy(st->bw_final, pl->d.bw, pl->g.N * 8, hipMemcpyDeviceToHost), "bw");
+  // :gradepro: bw_final of the ladder; :rot: the propagation time per axis
+  PKDE_TRY(hipMemcpy(st->bw_final, pl->kind == 1 ? pl->d.times : pl->d.bw, pl->g.N * 8,
+                     hipMemcpyDeviceToHost),
+           "bw");
   st->eps = par[0];
   st->steps_low = (int32_t)par[1];
   st->steps_over = (int32_t)par[2];

@@ -1461,6 +1461,30 @@ void launch_silverman(int N, const double* x, int64_t n, double prefactor, doubl
   k_moment_final<<<1, kThreads, 0, s>>>(N, kMomentBlocks, n, prefactor, partial, nullptr, bw_out);
 }
 
+// Rule-of-thumb propagation time of the :rot estimator (RoTEstimator.jl:
+// 22-28 Silverman, 42-47 Scott, 67-70): t_i = factor * sigma_i - h_i / 2,
+// factor = (n (N + 2) / 4)^(-1 / (N + 4)) (Silverman) or n^(-1 / (N + 4))
+// (Scott); sigma_i is the sample standard deviation (n - 1 normalisation).
+__global__ void k_rot_times(GridDev g, int64_t n, int rule, const double* __restrict__ sigma,
+                            double* __restrict__ times) {
+  if (threadIdx.x != 0) return;
+  const int N = g.N;
+  const double ex = __ddiv_rn(-1.0, (double)(N + 4));
+  const double base = rule == 1 ? __ddiv_rn((double)(n * (int64_t)(N + 2)), 4.0) : (double)n;
+  const double factor = pow(base, ex);
+  for (int i = 0; i < N; ++i)
+    times[i] = __dsub_rn(__dmul_rn(factor, sigma[i]), __ddiv_rn(g.h[i], 2.0));
+}
+
+void launch_rot_times(const GridDev& g, int64_t n, int rule, const double* sigma, double* times,
+                      hipStream_t s) {
+  k_rot_times<<<1, 64, 0, s>>>(g, n, rule, sigma, times);
+}
+
+template <typename T>
+void launch_gauss_tables(const GridDev& g, int S, const double* times, const double* kfreq, T* gM,
+                         T* gV, hipStream_t s);
+
 // ------------------------------------------------------------------- tables
 // Ladder scalars per rung (GradeProEstimator.jl:860-905, 954-966, 1074-1089;
 // DirectSpace.jl:564; FourierSpace.jl:224-225), with explicit rounding
@@ -1543,6 +1567,17 @@ void launch_tables(const GridDev& g, int S, int n_steps_mode, double n_steps_or_
   const int64_t total = (int64_t)S * g.ksum;
   k_gauss_tables<T><<<grid_for(total), kThreads, 0, s>>>(g, S, times, kfreq, gM, gV);
 }
+
+template <typename T>
+void launch_gauss_tables(const GridDev& g, int S, const double* times, const double* kfreq, T* gM,
+                         T* gV, hipStream_t s) {
+  const int64_t total = (int64_t)S * g.ksum;
+  k_gauss_tables<T><<<grid_for(total), kThreads, 0, s>>>(g, S, times, kfreq, gM, gV);
+}
+template void launch_gauss_tables<float>(const GridDev&, int, const double*, const double*, float*,
+                                         float*, hipStream_t);
+template void launch_gauss_tables<double>(const GridDev&, int, const double*, const double*,
+                                          double*, double*, hipStream_t);
 
 template void launch_tables<float>(const GridDev&, int, int, double, const double*, const double*,
                                    int64_t, double, double, double, double*, double*, double*,

@@ -99,6 +99,14 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
                         const double* cV, const double* vscale, const T* tw, const double* invk,
                         double* vmr, hipStream_t s);
 
+// :rot propagation time per axis from the sample standard deviations
+// (rule 0 Scott, 1 Silverman); and the per-axis Gaussian factor tables alone.
+void launch_rot_times(const GridDev& g, int64_t n, int rule, const double* sigma, double* times,
+                      hipStream_t s);
+template <typename T>
+void launch_gauss_tables(const GridDev& g, int S, const double* times, const double* kfreq, T* gM,
+                         T* gV, hipStream_t s);
+
 // Convergence state machine over all rungs -> chosen rung, selection rung.
 void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
                         const double* params /* eps, steps_low, steps_over */,

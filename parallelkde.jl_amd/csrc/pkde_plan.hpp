@@ -77,6 +77,8 @@ struct PlanBuffers {
 
 struct pkde_plan {
   pkde_ctx* ctx = nullptr;
+  int kind = 0;      // 0: :gradepro, 1: :rot (rule-of-thumb)
+  int rot_rule = 0;  // :rot: 0 Scott, 1 Silverman
   pkde::GridDev g{};
   pkde_gradepro_params p{};
   int B = 100;

@@ -28,7 +28,7 @@ __all__ = [
     "estimate_density", "estimate_density_", "get_density", "has_grid", "get_grid",
     "spacings", "bounds", "low_bounds", "high_bounds", "initial_bandwidth", "fftgrid",
     "AVAILABLE_DEVICES", "DEVICE_IMPLEMENTATIONS", "DEFAULT_IMPLEMENTATIONS",
-    "estimator_lookup", "GradeProResult", "estimate_batch", "Server", "context",
+    "estimator_lookup", "GradeProResult", "RoTResult", "estimate_batch", "Server", "context",
 ]
 
 
@@ -431,8 +431,78 @@ class GradeProEstimator:
               "seed", "precision", "deterministic")
 
 
+class RoTEstimator:
+    """Registry entry for :rot (RoTEstimator.jl:111).  `rule_of_thumb` is a
+    keyword of initialize_estimator there (default :scott, :120); here it is
+    a keyword of estimate_density."""
+    KWARGS = ("rule_of_thumb", "precision")
+    RULES = {"scott": 0, "silverman": 1}
+
+
 # DensityEstimators.jl:31-45
-estimator_lookup = {"gradepro": GradeProEstimator}
+estimator_lookup = {"gradepro": GradeProEstimator, "rot": RoTEstimator}
+
+
+@dataclass
+class RoTResult:
+    density: np.ndarray
+    time_propagated: np.ndarray  # rule-of-thumb bandwidth minus h/2, per axis
+    ms_total: float
+
+
+class _RoTPlan:
+    def __init__(self, grid: Grid, rule: int, precision: int):
+        L = _lib.load()
+        self.ctx = context()
+        N = grid.ndims
+        n = (_lib.C.c_int64 * N)(*grid.n)
+        lo = (_lib.C.c_double * N)(*grid.lo)
+        hi = (_lib.C.c_double * N)(*grid.hi)
+        h = (_lib.C.c_double * N)(*grid.h)
+        self.handle = _lib.C.c_void_p()
+        _raise(L.pkde_rot_plan_create(self.ctx.handle, N, n, lo, hi, h, rule, precision,
+                                      _lib.C.byref(self.handle)), "plan")
+        self.grid = grid
+
+    def estimate(self, x: np.ndarray) -> RoTResult:
+        L = _lib.load()
+        G = int(np.prod(self.grid.shape))
+        dens = np.empty(G)
+        st = _lib.Stats()
+        rc = L.pkde_gradepro_estimate(self.handle, x.ctypes.data, x.shape[0], None,
+                                      dens.ctypes.data, None, _lib.C.byref(st))
+        _raise(rc, "estimate")
+        return RoTResult(dens.reshape(self.grid.shape, order="F"),
+                         np.array(st.bw_final[: self.grid.ndims]), st.ms_total)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().pkde_gradepro_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _estimate_rot(est, rule_of_thumb="scott", precision="fp64"):
+    """RoTEstimator.jl:116-265 on the GPU."""
+    rule = str(rule_of_thumb).lstrip(":")
+    if rule not in RoTEstimator.RULES:
+        raise ArgumentError(f"Rule of thumb {rule} not implemented.")
+    prec = {"fp64": 1, "fp32": 0}.get(precision)
+    if prec is None:
+        raise ArgumentError("precision must be 'fp64' or 'fp32'")
+    pkey = ("rot", RoTEstimator.RULES[rule], prec)
+    plan = est._plans.get(pkey)
+    if plan is None:
+        plan = _RoTPlan(est.grid, RoTEstimator.RULES[rule], prec)
+        est._plans[pkey] = plan
+    res = plan.estimate(est.kde.data)
+    est.kde.density[...] = res.density
+    est.last_result = res
 
 
 def estimate_density(est: DensityEstimation, estimation_method="gradepro", method=None,
@@ -450,8 +520,15 @@ def estimate_density(est: DensityEstimation, estimation_method="gradepro", metho
         method = DEFAULT_IMPLEMENTATIONS[device]
     ensure_valid_implementation(device, method)
     if est.grid is None:
-        raise ArgumentError("the :gradepro estimator needs a grid")
-    unknown = set(kwargs) - set(GradeProEstimator.KWARGS)
+        raise ArgumentError(f"the :{key} estimator needs a grid")
+    unknown = set(kwargs) - set(estimator_lookup[key].KWARGS)
+    if unknown:
+        raise TypeError(f"unexpected keyword arguments {sorted(unknown)}")
+    if key == "rot":
+        if bootstrap_idx is not None:
+            raise TypeError("the :rot estimator takes no bootstrap_idx")
+        _estimate_rot(est, **kwargs)
+        return None
     if unknown:
         raise TypeError(f"unexpected keyword arguments {sorted(unknown)}")
     params = _params(est.grid.ndims, **kwargs)

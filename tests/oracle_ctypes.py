@@ -92,6 +92,8 @@ def lib():
                                            C.POINTER(OgParams), _dp, _i16p, C.c_void_p, C.c_void_p,
                                            C.c_int32, C.POINTER(OgInfo)]
         L.og_gradepro_estimate.restype = C.c_int
+        L.og_rot_estimate.argtypes = [C.POINTER(OgGrid), _dp, C.c_int64, C.c_int, _dp, _dp]
+        L.og_rot_estimate.restype = C.c_int
         _lib = L
     return _lib
 
@@ -177,6 +179,16 @@ def dirac_sequences(g: OgGrid, samples_col: np.ndarray, idx_bs: np.ndarray, with
     rc = lib().og_dirac_sequences(C.byref(g), s.ravel(), n, idx.ravel(), B, sums,
                                   sqs.ctypes.data if sqs is not None else None, threads)
     return rc, sums.reshape(B, G), (sqs.reshape(B, G) if sqs is not None else None)
+
+
+def rot(g: OgGrid, samples_col, rule=0):
+    """og_rot_estimate: (rc, density [G] column-major, propagation time [N])."""
+    s = np.ascontiguousarray(samples_col, dtype=np.float64)
+    G = int(np.prod(grid_shape(g)))
+    dens = np.zeros(G)
+    t = np.zeros(g.ndims)
+    rc = lib().og_rot_estimate(C.byref(g), s.ravel(), s.shape[0], int(rule), dens, t)
+    return rc, dens, t
 
 
 @dataclass

@@ -344,3 +344,37 @@ def test_server_alone_equals_in_mix(pk):
     est = pk.initialize_estimation(x, grid=g)
     pk.estimate_density(est, "gradepro", **kw)
     assert np.array_equal(alone[0][0], pk.get_density(est))
+
+
+# ------------------------------------------------------------ :rot estimator
+ROT_CASES = [
+    ("1d-512", lambda: W.gaussian_1d(10000, 3), ([(-5.0, 5.0, 512)])),
+    ("2d-128", lambda: W.mixture_2d(20000, 4), ([(-6.0, 6.0, 128)] * 2)),
+    ("3d-32", lambda: W.correlated_3d(5000, 5), ([(-4.0, 4.0, 32)] * 3)),
+]
+
+
+@pytest.mark.parametrize("case", ROT_CASES, ids=[c[0] for c in ROT_CASES])
+@pytest.mark.parametrize("rule", ["scott", "silverman"])
+def test_rot_matches_oracle(pk, case, rule):
+    """RoTEstimator.jl:116-265 through the C ABI against the FP64 oracle:
+    propagation time and density (FP64: relative L1 <= 1e-12)."""
+    name, gen, ranges = case
+    say(f"[rot] {name} {rule}")
+    x = gen()
+    est = pk.initialize_estimation(x, grid=True, grid_ranges=ranges)
+    pk.estimate_density(est, "rot", rule_of_thumb=rule)
+    res = est.last_result
+    g = O.make_grid([r[0] for r in ranges], [r[1] for r in ranges], [r[2] for r in ranges])
+    rc, dens, t = O.rot(g, x, 0 if rule == "scott" else 1)
+    assert rc == 0
+    assert np.allclose(res.time_propagated, t, rtol=1e-13, atol=0)
+    gpu = pk.get_density(est).ravel(order="F")
+    rel = np.abs(gpu - dens).sum() / np.abs(dens).sum()
+    say(f"  rel L1 {rel:.3e}")
+    assert rel <= 1e-12
+    # FP32 spectra: within single-precision rounding
+    pk.estimate_density(est, "rot", rule_of_thumb=rule, precision="fp32")
+    rel32 = np.abs(pk.get_density(est).ravel(order="F") - dens).sum() / np.abs(dens).sum()
+    say(f"  fp32 rel L1 {rel32:.3e}")
+    assert rel32 <= 1e-5

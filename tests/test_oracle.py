@@ -399,3 +399,77 @@ def test_chosen_rung_consistent_with_means_table():
     S = r.info.n_rungs
     sel = np.where(r.chosen == S, S - 1, r.chosen)
     assert np.array_equal(r.density, r.means[sel, np.arange(128)])
+
+
+# ------------------------------------------------------- :rot estimator
+def _mise(f1, f2, dx):
+    """calculate_mise (test/test_utils.jl:24-28)."""
+    return ((f1 - f2) ** 2).sum() * dx / f1.size
+
+
+@pytest.mark.parametrize("n_dims", [1, 2])
+@pytest.mark.parametrize("rule", [0, 1])
+def test_rot_mise_reference_test(n_dims, rule):
+    """rulesofthumb_tests.jl:96-126: 1000 N(0,I) samples, grid
+    range(-6, 6, length=300) per axis, MISE < 1e-3 against the true pdf
+    (seeded here; both rules)."""
+    from scipy.stats import norm
+    rng = np.random.default_rng(60 + n_dims)
+    x = rng.standard_normal((1000, n_dims))
+    g = O.make_grid([-6.0] * n_dims, [6.0] * n_dims, [300] * n_dims)
+    rc, dens, t = O.rot(g, x, rule)
+    assert rc == 0
+    xs = np.linspace(-6, 6, 300)
+    truth = norm.pdf(xs)
+    if n_dims == 2:
+        truth = np.outer(truth, truth).ravel(order="F")
+    dx = (xs[1] - xs[0]) ** n_dims
+    assert _mise(dens, truth, dx) < 1e-3
+    assert np.all(np.isfinite(dens))
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_rot_propagation_time(n_dims):
+    """propagation_time (RoTEstimator.jl:22-28, 42-47) minus the initial
+    bandwidth h/2 (RoTEstimator.jl:67-70), against numpy's ddof=1 standard
+    deviation: Scott n^(-1/(N+4)), Silverman (n (N+2)/4)^(-1/(N+4))."""
+    rng = np.random.default_rng(70 + n_dims)
+    n = 777
+    x = rng.standard_normal((n, n_dims)) * np.array([1.0, 0.5, 2.0][:n_dims]) + 0.3
+    g = O.make_grid([-9.0] * n_dims, [9.0] * n_dims, [32] * n_dims)
+    sd = x.std(axis=0, ddof=1)
+    h = np.array([g.h[i] for i in range(n_dims)])
+    for rule, base in ((0, float(n)), (1, n * (n_dims + 2) / 4.0)):
+        rc, _, t = O.rot(g, x, rule)
+        assert rc == 0
+        assert np.allclose(t, base ** (-1.0 / (n_dims + 4)) * sd - h / 2, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_rot_matches_numpy_pipeline(n_dims):
+    """The oracle's :rot after the Dirac sequence (pinned by the binning KAT)
+    against numpy's FFT: |ifftn(fftn(f) * exp(-sum (k_i t_i)^2 / 2))| with
+    k = 2 pi fftfreq(n, h) (FourierSpace.jl:237-262; RoTEstimator.jl:
+    237-265), odd and even axis lengths."""
+    rng = np.random.default_rng(80 + n_dims)
+    x = np.clip(rng.standard_normal((500, n_dims)), -3.0, 3.0)
+    shape = [21, 16, 12][:n_dims]
+    g = O.make_grid([-5.0] * n_dims, [5.0] * n_dims, shape)
+    rc, dens, t = O.rot(g, x, 1)
+    assert rc == 0
+    _, f, _ = O.dirac_sequences(g, x, np.arange(500, dtype=np.int32)[None, :])
+    F = np.fft.fftn(f[0].reshape(shape, order="F"))
+    expo = np.zeros(shape)
+    for i in range(n_dims):
+        k = 2 * np.pi * np.fft.fftfreq(shape[i], g.h[i])
+        sh = [1] * n_dims
+        sh[i] = shape[i]
+        expo = expo + (k * t[i]).reshape(sh) ** 2
+    ref = np.abs(np.fft.ifftn(F * np.exp(-0.5 * expo))).ravel(order="F")
+    assert np.allclose(dens, ref, rtol=1e-11, atol=1e-13 * ref.max())
+
+
+def test_rot_out_of_grid():
+    g = O.make_grid([-1.0], [1.0], [16])
+    rc, _, _ = O.rot(g, np.array([[0.0], [2.0]]), 0)
+    assert rc == 2
