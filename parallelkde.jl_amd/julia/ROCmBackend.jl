@@ -190,4 +190,54 @@ function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N
     return nothing
 end
 
+# ------------------------------------------------------------------- :rot
+# RoTEstimator.jl:116-265 on the device: one plan per (grid, rule).
+mutable struct RocRoTEstimator{N} <: DensityEstimators.AbstractRoTEstimator
+    plan::Ptr{Cvoid}
+end
+
+function destroy!(e::RocRoTEstimator)
+    if e.plan != C_NULL
+        ccall((:pkde_gradepro_plan_destroy, libpkde), Cvoid, (Ptr{Cvoid},), e.plan)
+        e.plan = C_NULL
+    end
+    return nothing
+end
+
+function DensityEstimators.initialize_estimator(
+    ::Type{<:DensityEstimators.AbstractRoTEstimator},
+    kde::RocKDE{N};
+    method::Symbol,
+    grid::Union{RocGrid,Nothing}=nothing,
+    rule_of_thumb::Union{Nothing,Symbol}=:scott,
+    precision::Symbol=:fp64,
+    kwargs...,
+) where {N}
+    grid === nothing && throw(ArgumentError("KDE device IsROCm() does not match Grid device nothing."))
+    rule = rule_of_thumb === :silverman ? Cint(1) :
+           rule_of_thumb === :scott ? Cint(0) :
+           throw(ArgumentError("Rule of thumb $rule_of_thumb not implemented."))
+    g = grid.grid
+    n = Int64[length(g.coordinates[i]) for i in 1:N]
+    lo = Float64[Grids.low_bounds(g)[i] for i in 1:N]
+    hi = Float64[Grids.high_bounds(g)[i] for i in 1:N]
+    h = Float64[Grids.spacings(g)[i] for i in 1:N]
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pkde_rot_plan_create, libpkde), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cint, Ref{Ptr{Cvoid}}),
+        context(), N, n, lo, hi, h, rule, Cint(precision === :fp64 ? 1 : 0), plan))
+    return finalizer(destroy!, RocRoTEstimator{N}(plan[]))
+end
+
+function DensityEstimators.estimate!(est::RocRoTEstimator{N}, kde::RocKDE{N}; method::Symbol, kwargs...) where {N}
+    est.plan == C_NULL && throw(ArgumentError("estimator plan was destroyed"))
+    n = size(kde.data, 2)
+    GC.@preserve est kde begin
+        check(ccall((:pkde_gradepro_estimate, libpkde), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Int16}, Ptr{Cvoid}),
+            est.plan, kde.data, n, C_NULL, kde.density, C_NULL, C_NULL))
+    end
+    return nothing
+end
+
 end # module
