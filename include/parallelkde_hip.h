@@ -94,8 +94,9 @@ void pkde_context_destroy(pkde_ctx *ctx);
 int pkde_device_free_memory(pkde_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* A plan binds a grid (Grid: Grids.jl:67-87 -- per-axis lo, hi, spacing and
- * point count) and the estimator keywords.  Axis sizes must be powers of two
- * in [2, 4096]; ndims in 1..3 (the reference's GPU path is capped at 3-D,
+ * point count) and the estimator keywords.  Axis sizes must lie in [2, 4096]
+ * and have no prime factor above 13 (the radices of the mixed-radix FFT;
+ * others return PKDE_UNSUPPORTED); ndims in 1..3 (the reference's GPU path is capped at 3-D,
  * GradeProEstimator.jl:811-814).  Replaces initialize_estimator
  * (GradeProEstimator.jl:700-735) for the :rocm device. */
 int pkde_gradepro_plan_create(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,

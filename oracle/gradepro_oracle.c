@@ -277,6 +277,9 @@ typedef struct {
   double *w; /* w[2k], w[2k+1] = cos, sin of 2 pi k / n */
 } og_fft1;
 
+static int og_fft_reversed = 0;
+void og_set_fft_order(int reversed) { og_fft_reversed = reversed ? 1 : 0; }
+
 static int og_fft1_init(og_fft1 *p, int64_t n) {
   p->n = n;
   p->nf = 0;
@@ -286,6 +289,12 @@ static int og_fft1_init(og_fft1 *p, int64_t n) {
   for (int64_t f = 3; f * f <= m; f += 2)
     while (m % f == 0) { p->factors[p->nf++] = f; m /= f; }
   if (m > 1) p->factors[p->nf++] = m;
+  if (og_fft_reversed)
+    for (int i = 0, j = p->nf - 1; i < j; ++i, --j) {
+      const int64_t t = p->factors[i];
+      p->factors[i] = p->factors[j];
+      p->factors[j] = t;
+    }
   p->w = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
   if (!p->w) return OG_OUT_OF_MEMORY;
   for (int64_t k = 0; k < n; ++k) {

@@ -92,6 +92,13 @@ int og_dirac_sequences(const og_grid *g, const double *samples, int64_t n,
  * (re, im).  Stands in for FFTW's plan_fft!/plan_ifft! (FourierSpace.jl:25-52,
  * 445-472); the inverse here is NOT scaled by 1/G. */
 int og_fft_nd(int ndims, const int64_t *n, double *data, int sign);
+/* Selects the radix order of og_fft_nd for the calling process: 0 (default)
+ * applies radix 4, then 2, then odd primes ascending; 1 applies the same
+ * factors in reverse order.  Both are exact DFTs in exact arithmetic; the
+ * second sums in a different order, so running a whole estimate under each
+ * measures how far FP64 rounding alone moves the reference's result (its
+ * rounding envelope).  Not thread-safe: set it between estimates. */
+void og_set_fft_order(int reversed);
 /* propagate_time_cpu! (FourierSpace.jl:215-262) on one complex field. vars may
  * be NULL (means-only form). */
 void og_propagate(const og_grid *g, const double *kgrid_axes, const double *t,

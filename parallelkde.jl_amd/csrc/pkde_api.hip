@@ -49,6 +49,20 @@ static void dfree(void* p) {
 
 static bool is_pow2(int64_t v) { return v >= 2 && (v & (v - 1)) == 0; }
 
+// Radix list of a 13-smooth length: 16s, one 8/4/2, then 3, 5, 7, 11, 13.
+static bool factorize(int L, FftFactors& f) {
+  f.L = L;
+  f.np = 0;
+  int n = L;
+  while (n % 16 == 0 && f.np < 16) f.rad[f.np++] = 16, n /= 16;
+  if (n % 8 == 0) f.rad[f.np++] = 8, n /= 8;
+  else if (n % 4 == 0) f.rad[f.np++] = 4, n /= 4;
+  else if (n % 2 == 0) f.rad[f.np++] = 2, n /= 2;
+  for (int p : {3, 5, 7, 11, 13})
+    while (n % p == 0 && f.np < 16) f.rad[f.np++] = p, n /= p;
+  return n == 1;
+}
+
 static int ilog2(int64_t v) {
   int l = 0;
   while ((1ll << l) < v) ++l;
@@ -68,7 +82,7 @@ static int frac_bits_for(int64_t n) {
 static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; }
 
 // The rung-chunk buffer has y_units() equal parts (two in 3-D: the axis-2
-// result and the transposed axis-3 result); each part holds at least one
+// result and the axis-3 result in the tiled row layout); each part holds at least one
 // rung of the 2B bootstrap fields.
 static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
 
@@ -177,7 +191,7 @@ static void plan_free(pkde_plan* pl) {
   PlanBuffers& d = pl->d;
   void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.chosen, d.sel, d.density, d.err,
                 d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
-                d.gM, d.gV, d.invk, d.binwork};
+                d.gM, d.gV, d.invk, d.binwork, d.twg64, d.twg32};
   for (void* p : ps) dfree(p);
   d = PlanBuffers();
   for (auto& e : pl->ev)
@@ -194,8 +208,10 @@ static int enqueue_means(pkde_plan* pl, const cplx<T>* spec, const T* gM, const 
   const GridDev& g = pl->g;
   const int B = pl->B, S = pl->S, N = g.N;
   const int64_t fs = fsize(pl);
+  const T* twg = (const T*)(sizeof(T) == 8 ? (const void*)d.twg64 : (const void*)d.twg32);
   if (N == 1) {
-    launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, d.sel, d_density, s);
+    launch_means_select<T>(g, spec + 2 * (int64_t)B * fs, 0, S, 0, S, gM, tw, twg, d.sel, d_density,
+                           s);
   } else {
     if (d.Y_elems < 2 * fs) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than two rungs");
     int Rm = (int)std::min<int64_t>(S, d.Y_elems / fs);
@@ -204,10 +220,11 @@ static int enqueue_means(pkde_plan* pl, const cplx<T>* spec, const T* gM, const 
     const int fM = 2 * B;
     for (int r0 = 0; r0 < S; r0 += Rm) {
       const int R = std::min(Rm, S - r0);
-      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
+      launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, twg, s);
       if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, s);
-      launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, d.sel, d_density, s);
+        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, twg,
+                                 s);
+      launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, twg, d.sel, d_density, s);
     }
   }
   return PKDE_OK;
@@ -221,6 +238,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   const int B = pl->B, S = pl->S, N = g.N;
   const int64_t G = g.G, F = 2 * (int64_t)B + 1, fs = fsize(pl);
   const T* tw = (const T*)(sizeof(T) == 8 ? (const void*)pl->ctx->tw64 : (const void*)pl->ctx->tw32);
+  const T* twg = (const T*)(sizeof(T) == 8 ? (const void*)d.twg64 : (const void*)d.twg32);
 
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
   launch_fill_i32(d.err, 0, 1, s);
@@ -248,11 +266,12 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const double two_fb = ldexp(1.0, frac_bits_for(n));
     const double scaleM = (double)n * hp * two_fb;
     const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
-    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, B, d.specD, pl->ctx->tw64, s);
+    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, B, d.specD, pl->ctx->tw64, d.twg64,
+                         s);
   }
   for (int ax = 1; ax < N; ++ax)
     launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
-                                        nullptr, nullptr, 0, pl->ctx->tw64, s);
+                                        nullptr, nullptr, 0, pl->ctx->tw64, d.twg64, s);
   const cplx<T>* spec;
   if (sizeof(T) == 4) {
     launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
@@ -266,16 +285,16 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
   const T* gM = (const T*)d.gM;
   const T* gV = (const T*)d.gV;
   if (N == 1) {
-    launch_ladder_vmr<T>(g, spec, 0, B, n, S, 0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+    launch_ladder_vmr<T>(g, spec, 0, B, n, S, 0, gM, gV, d.cV, d.vscale, tw, twg, d.vmr, s);
   } else {
-    // axis 2 (and 3) inverse passes write the transposed layout [q][k1] the
-    // row kernel reads; in 3-D the axis-2 result goes through a second buffer
+    // axis 2 (and 3) inverse passes write the tiled row layout the row kernel
+    // reads; in 3-D the axis-2 result goes through a second buffer
     const int64_t per_rung = 2 * (int64_t)B * g.fsY;  // tiled rows (>= H1 * Q per field)
     const int Ew = sizeof(T) == 8 ? 8 : 16;  // wave-local FFT range [E, 64 E]
     auto wave_ok = [&](int L) { return L >= Ew && L <= 64 * Ew; };
     // the row kernel needs the axes before it in the wave-local range (their
     // passes write the tiled row layout)
-    const bool rows = g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
+    const bool rows = g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
                       (N == 2 || wave_ok(g.n[2]));
     const bool rows3 = N == 3 && rows;
     const int64_t cap = d.Y_elems / y_units(pl);
@@ -293,12 +312,12 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
-                                 s);
+                                 twg, s);
       }
       if (N == 3) {
         if (!rows3)
           launch_axis_fft<T, T, 1>(g, 2, Y2, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV,
-                                   B, tw, s);
+                                   B, tw, twg, s);
         else if (!launch_axis3_rows<T>(g, Y2, per_rung, Y, per_rung, 2 * B, R, r0, gM, gV, B, tw,
                                        s))
           return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
@@ -307,7 +326,8 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
         launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
                               d.vmr, s);
       else
-        launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.vmr, s);
+        launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, twg, d.vmr,
+                             s);
     }
   }
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
@@ -355,11 +375,12 @@ static int enqueue_rot(pkde_plan* pl, const double* d_x, int64_t n, double* d_de
     const double two_fb = ldexp(1.0, frac_bits_for(n));
     const double scaleM = (double)n * hp * two_fb;
     const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
-    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, 0, d.specD, pl->ctx->tw64, s);
+    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, 0, d.specD, pl->ctx->tw64, d.twg64,
+                         s);
   }
   for (int ax = 1; ax < N; ++ax)
     launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, 1, 1, 0, nullptr,
-                                        nullptr, 0, pl->ctx->tw64, s);
+                                        nullptr, 0, pl->ctx->tw64, d.twg64, s);
   const cplx<T>* spec;
   if (sizeof(T) == 4) {
     launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, fs, s);
@@ -472,6 +493,12 @@ void pkde_context_destroy(pkde_ctx* ctx) {
     if (ln.stream) (void)hipStreamDestroy(ln.stream);
   }
   ctx->lanes.clear();
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  {
+    std::lock_guard<std::mutex> lk(ctx->live_mu);
+    for (pkde_plan* pl : ctx->live) pl->ctx = nullptr;  // the caller still owns them
+    ctx->live.clear();
+  }
   if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->tw64) (void)hipFree(ctx->tw64);
@@ -517,17 +544,20 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     g.lo[i] = g.hi[i] = g.h[i] = 0.0;
     g.koff[i] = 0;
   }
+  g.pow2 = 1;
   for (int i = 0; i < ndims; ++i) {
-    if (!is_pow2(n_per_axis[i]) || n_per_axis[i] > 4096) {
+    if (n_per_axis[i] < 2 || n_per_axis[i] > 4096 || !factorize((int)n_per_axis[i], g.fac[i])) {
       delete pl;
-      return fail(PKDE_UNSUPPORTED, "grid axis sizes must be powers of two in [2, 4096]");
+      return fail(PKDE_UNSUPPORTED,
+                  "grid axis sizes must be in [2, 4096] with no prime factor above 13");
     }
+    if (!is_pow2(n_per_axis[i])) g.pow2 = 0;
     if (!(h[i] > 0) || !(hi[i] > lo[i]) || !isfinite(lo[i]) || !isfinite(hi[i])) {
       delete pl;
       return fail(PKDE_INVALID_ARGUMENT, "invalid grid bounds or spacing");
     }
     g.n[i] = (int)n_per_axis[i];
-    g.log2n[i] = ilog2(n_per_axis[i]);
+    g.log2n[i] = is_pow2(n_per_axis[i]) ? ilog2(n_per_axis[i]) : 0;
     g.lo[i] = lo[i];
     g.hi[i] = hi[i];
     g.h[i] = h[i];
@@ -610,6 +640,24 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     const int64_t nn = (ni + 1) >> 1;
     for (int64_t j = 0; j < ni; ++j) kf[g.koff[i] + j] = (double)(j < nn ? j : j - ni) * mult;
   }
+  // per-axis twiddles exp(-2 pi i k / n_i) for the mixed-radix kernels
+  {
+    std::vector<double> t64(2 * (size_t)g.ksum);
+    std::vector<float> t32(2 * (size_t)g.ksum);
+    for (int i = 0; i < ndims; ++i)
+      for (int k = 0; k < g.n[i]; ++k) {
+        const double a = 6.283185307179586476925286766559 * (double)k / (double)g.n[i];
+        const double cr = cos(a), ci = -sin(a);
+        t64[2 * (size_t)(g.koff[i] + k)] = cr;
+        t64[2 * (size_t)(g.koff[i] + k) + 1] = ci;
+        t32[2 * (size_t)(g.koff[i] + k)] = (float)cr;
+        t32[2 * (size_t)(g.koff[i] + k) + 1] = (float)ci;
+      }
+    if (!rc) rc = dalloc((void**)&pl->d.twg64, t64.size() * 8, "twiddles");
+    if (!rc) rc = dalloc((void**)&pl->d.twg32, t32.size() * 4, "twiddles");
+    if (!rc) rc = check_hip(hipMemcpy(pl->d.twg64, t64.data(), t64.size() * 8, hipMemcpyHostToDevice), "twiddles");
+    if (!rc) rc = check_hip(hipMemcpy(pl->d.twg32, t32.data(), t32.size() * 4, hipMemcpyHostToDevice), "twiddles");
+  }
   if (!rc) rc = dalloc((void**)&pl->d.kfreq, kf.size() * 8, "kfreq");
   if (!rc) rc = check_hip(hipMemcpy(pl->d.kfreq, kf.data(), kf.size() * 8, hipMemcpyHostToDevice), "kfreq");
   if (!rc) rc = dalloc((void**)&pl->d.bw, 256, "bw");
@@ -620,6 +668,10 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     plan_free(pl);
     delete pl;
     return rc;
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->live_mu);
+    ctx->live.push_back(pl);
   }
   *out = pl;
   return PKDE_OK;
@@ -646,14 +698,21 @@ int pkde_rot_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
 
 void pkde_gradepro_plan_destroy(pkde_plan* plan) {
   if (!plan) return;
-  (void)hipSetDevice(plan->ctx->device);
-  (void)hipStreamSynchronize(plan->ctx->stream);
+  if (pkde_ctx* ctx = plan->ctx) {
+    {
+      std::lock_guard<std::mutex> lk(ctx->live_mu);
+      auto it = std::find(ctx->live.begin(), ctx->live.end(), plan);
+      if (it != ctx->live.end()) ctx->live.erase(it);
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
   plan_free(plan);
   delete plan;
 }
 
 uint64_t pkde_gradepro_plan_footprint(const pkde_plan* plan, int64_t n_samples) {
-  return plan ? plan_bytes(plan, n_samples) : 0;
+  return (plan && plan->ctx) ? plan_bytes(plan, n_samples) : 0;
 }
 
 static int finish_stats(pkde_plan* pl, int64_t n, pkde_stats* st) {
@@ -682,6 +741,7 @@ static int finish_stats(pkde_plan* pl, int64_t n, pkde_stats* st) {
 int pkde_gradepro_estimate(pkde_plan* pl, const double* samples, int64_t n, const int32_t* bootstrap_idx,
                            double* density_out, int16_t* chosen_rung_out, pkde_stats* stats) {
   if (!pl || !samples || !density_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
   std::lock_guard<std::mutex> lk(pl->ctx->mu);
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
@@ -710,6 +770,7 @@ int pkde_gradepro_estimate_device(pkde_plan* pl, const double* d_samples, int64_
                                   const int32_t* d_bootstrap_idx, double* d_density,
                                   int16_t* d_chosen_rung, void* stream) {
   if (!pl || !d_samples || !d_density) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   hipStream_t s = stream ? (hipStream_t)stream : pl->ctx->stream;
   return plan_enqueue(pl, d_samples, n, d_bootstrap_idx, d_density, d_chosen_rung, s, true);
@@ -717,6 +778,7 @@ int pkde_gradepro_estimate_device(pkde_plan* pl, const double* d_samples, int64_
 
 int pkde_gradepro_check(pkde_plan* pl) {
   if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   if (!pl->d.err || pl->last_n == 0) return fail(PKDE_INVALID_ARGUMENT, "no estimate has run");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
@@ -729,6 +791,7 @@ int pkde_gradepro_check(pkde_plan* pl) {
 
 int pkde_gradepro_last_stats(pkde_plan* pl, pkde_stats* stats) {
   if (!pl || !stats) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
   return finish_stats(pl, pl->last_n, stats);
@@ -745,6 +808,7 @@ int pkde_gradepro_debug_vmr(pkde_plan* pl, double* vmr_out, int64_t capacity) {
   const int64_t need = (int64_t)pl->S * pl->g.G;
   if (capacity < need) return fail(PKDE_INVALID_ARGUMENT, "vmr buffer too small");
   if (!pl->d.vmr) return fail(PKDE_INVALID_ARGUMENT, "no estimate has run");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   PKDE_TRY(hipStreamSynchronize(pl->ctx->stream), "sync");
   PKDE_TRY(hipMemcpy(vmr_out, pl->d.vmr, (size_t)need * 8, hipMemcpyDeviceToHost), "vmr");

@@ -180,6 +180,120 @@ __device__ void lds_fft(cplx<T>* buf, int log2L, int npen, int pitch,
   }
 }
 
+// ------------------------------------------------------------ mixed radix
+// Pencil length L = prod(rad[0..np)) with radices in {2, 3, 4, 5, 7, 8, 11,
+// 13, 16} (13-smooth lengths); tw[k] = exp(-2 pi i k / L), k < L.
+struct FftFactors {
+  int L, np;
+  int rad[16];
+};
+
+// v[s] = sum_r v[r] exp(DIR 2 pi i r s / R) for an odd prime R, with the
+// roots taken from the length-L table: exp(-2 pi i m / R) = tw[m * (L / R)].
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft_prime(cplx<T>* v, const cplx<T>* __restrict__ tw, int LoverR) {
+  cplx<T> out[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    cplx<T> acc = v[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      cplx<T> f = tw[((r * s) % R) * LoverR];
+      if (DIR > 0) f.im = -f.im;
+      acc = cadd(acc, cmul(v[r], f));
+    }
+    out[s] = acc;
+  }
+#pragma unroll
+  for (int s = 0; s < R; ++s) v[s] = out[s];
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft_any(cplx<T>* v, const cplx<T>* __restrict__ tw, int L) {
+  if constexpr (R == 2 || R == 4 || R == 8 || R == 16)
+    dft_reg<R, DIR, T>(v);
+  else
+    dft_prime<R, DIR, T>(v, tw, L / R);
+}
+
+// One mixed-radix Stockham pass over npen pencils of length L at
+// buf[p * pitch + e]: npen * L / R butterflies, at most BPT per thread.  All
+// inputs are read before the barrier, so the pass is done in place; the
+// caller keeps npen * L <= BPT * R * nthr (npen * L <= 4096 with 256 threads).
+template <int R, int BPT, int DIR, typename T>
+__device__ __forceinline__ void stockham_pass_gen(cplx<T>* buf, int L, int npen, int pitch, int Ns,
+                                                  const cplx<T>* __restrict__ tw, int tid,
+                                                  int nthr) {
+  const int nb = L / R;
+  const int items = npen * nb;
+  const int twstep = L / (Ns * R);
+  cplx<T> v[BPT][R];
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const int it = tid + k * nthr;
+    if (it < items) {
+      const int pen = it / nb, j = it - pen * nb;
+      const cplx<T>* pb = buf + (int64_t)pen * pitch;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = pb[j + r * nb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const int it = tid + k * nthr;
+    if (it < items) {
+      const int pen = it / nb, j = it - pen * nb;
+      const int jm = j % Ns;
+      cplx<T>* w = v[k];
+      if (jm != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> f = tw[jm * r * twstep];
+          if (DIR > 0) f.im = -f.im;
+          w[r] = cmul(w[r], f);
+        }
+      }
+      dft_any<R, DIR, T>(w, tw, L);
+      cplx<T>* pb = buf + (int64_t)pen * pitch;
+      const int out0 = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pb[out0 + r * Ns] = w[r];
+    }
+  }
+  __syncthreads();
+}
+
+// Batched in-LDS FFT of any 13-smooth length f.L <= 4096: natural order in
+// and out, unnormalised; DIR = -1 forward, +1 inverse.  Pencils are done in
+// groups of at most 4096 points so every pass fits one round of 256 threads.
+// Must be called by the whole workgroup of 256 threads.
+template <int DIR, typename T>
+__device__ void lds_fft_gen(cplx<T>* buf, const FftFactors& f, int npen, int pitch,
+                            const cplx<T>* __restrict__ tw, int tid, int nthr) {
+  const int L = f.L;
+  const int pg = L <= 4096 ? 4096 / L : 1;
+  for (int p0 = 0; p0 < npen; p0 += pg) {
+    cplx<T>* b = buf + (int64_t)p0 * pitch;
+    const int cnt = npen - p0 < pg ? npen - p0 : pg;
+    int Ns = 1;
+    for (int p = 0; p < f.np; ++p) {
+      switch (f.rad[p]) {
+        case 2: stockham_pass_gen<2, 8, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 3: stockham_pass_gen<3, 6, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 4: stockham_pass_gen<4, 4, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 5: stockham_pass_gen<5, 4, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 7: stockham_pass_gen<7, 3, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 8: stockham_pass_gen<8, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 11: stockham_pass_gen<11, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 13: stockham_pass_gen<13, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        default: stockham_pass_gen<16, 1, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+      }
+      Ns *= f.rad[p];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon et al., SC'11): counter-based bootstrap draws.
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

@@ -206,7 +206,9 @@ __device__ __forceinline__ double acc_value(const void* acc, int f, int B, int64
   return ((const double*)acc)[(int64_t)f * G + off];
 }
 
-template <bool FIXED>
+// GEN: mixed-radix FFT with the plan's twiddle table (axis length not a power
+// of two); otherwise the power-of-two FFT with the context table.
+template <bool FIXED, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_forward_axis1(GridDev g, const void* __restrict__ acc,
                                                              double invM, double invV, int B, int TQ,
                                                              cplx<double>* __restrict__ specD,
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_axis1(GridDev g, const voi
   cplx<double>* z = (cplx<double>*)smem;
   const int n1 = g.n[0], H1 = g.H1;
   const int64_t Q = g.Q;
-  const int ntile = (int)(Q / TQ);
+  const int ntile = (int)((Q + TQ - 1) / TQ);
   const int pi = blockIdx.x / ntile;
   const int64_t q0 = (int64_t)(blockIdx.x % ntile) * TQ;
   int fa, fb;
@@ -224,16 +226,23 @@ __global__ __launch_bounds__(kThreads) void k_forward_axis1(GridDev g, const voi
   for (int id = tid; id < TQ * n1; id += nthr) {
     const int t = id / n1, j = id - t * n1;
     const int64_t off = (q0 + t) * n1 + j;
-    const double va = acc_value<FIXED>(acc, fa, B, g.G, off, invM, invV);
-    const double vb = fb >= 0 ? acc_value<FIXED>(acc, fb, B, g.G, off, invM, invV) : 0.0;
+    double va = 0.0, vb = 0.0;
+    if (q0 + t < Q) {
+      va = acc_value<FIXED>(acc, fa, B, g.G, off, invM, invV);
+      vb = fb >= 0 ? acc_value<FIXED>(acc, fb, B, g.G, off, invM, invV) : 0.0;
+    }
     z[t * n1 + j] = {va, vb};
   }
   __syncthreads();
-  lds_fft<-1, double>(z, g.log2n[0], TQ, n1, tw + (n1 - 2), tid, nthr);
+  if constexpr (GEN)
+    lds_fft_gen<-1, double>(z, g.fac[0], TQ, n1, tw, tid, nthr);
+  else
+    lds_fft<-1, double>(z, g.log2n[0], TQ, n1, tw + (n1 - 2), tid, nthr);
   for (int id = tid; id < TQ * H1; id += nthr) {
     const int k = id / TQ, t = id - k * TQ;
+    if (q0 + t >= Q) continue;
     const cplx<double> Zk = z[t * n1 + k];
-    const cplx<double> Zm = z[t * n1 + ((n1 - k) & (n1 - 1))];
+    const cplx<double> Zm = z[t * n1 + (k == 0 ? 0 : n1 - k)];
     const cplx<double> A = {0.5 * (Zk.re + Zm.re), 0.5 * (Zk.im - Zm.im)};
     const cplx<double> Bv = {0.5 * (Zk.im + Zm.im), -0.5 * (Zk.re - Zm.re)};
     const int64_t o = (int64_t)k * Q + q0 + t;
@@ -244,20 +253,25 @@ __global__ __launch_bounds__(kThreads) void k_forward_axis1(GridDev g, const voi
 
 void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double scaleM,
                           double scaleV, int B, cplx<double>* specD, const double* tw64,
-                          hipStream_t s) {
+                          const double* twg, hipStream_t s) {
   const int n1 = g.n[0];
   int TQ = 2048 / n1;
   if (TQ < 1) TQ = 1;
   if (TQ > g.Q) TQ = (int)g.Q;
   const int npairs = (B + 2) / 2 + (B + 1) / 2;
-  const int blocks = npairs * (int)(g.Q / TQ);
+  const int blocks = npairs * (int)((g.Q + TQ - 1) / TQ);
   const size_t lds = (size_t)TQ * n1 * sizeof(cplx<double>);
-  const cplx<double>* tw = (const cplx<double>*)tw64;
-  if (fixed)
-    k_forward_axis1<true><<<blocks, kThreads, lds, s>>>(g, acc, 1.0 / scaleM, 1.0 / scaleV, B, TQ,
-                                                        specD, tw);
+  const bool gen = (n1 & (n1 - 1)) != 0;
+  const cplx<double>* tw = (const cplx<double>*)(gen ? twg + 2 * g.koff[0] : tw64);
+  const double iM = fixed ? 1.0 / scaleM : 1.0, iV = fixed ? 1.0 / scaleV : 1.0;
+  if (fixed && gen)
+    k_forward_axis1<true, true><<<blocks, kThreads, lds, s>>>(g, acc, iM, iV, B, TQ, specD, tw);
+  else if (fixed)
+    k_forward_axis1<true, false><<<blocks, kThreads, lds, s>>>(g, acc, iM, iV, B, TQ, specD, tw);
+  else if (gen)
+    k_forward_axis1<false, true><<<blocks, kThreads, lds, s>>>(g, acc, iM, iV, B, TQ, specD, tw);
   else
-    k_forward_axis1<false><<<blocks, kThreads, lds, s>>>(g, acc, 1.0, 1.0, B, TQ, specD, tw);
+    k_forward_axis1<false, false><<<blocks, kThreads, lds, s>>>(g, acc, iM, iV, B, TQ, specD, tw);
 }
 
 // ------------------------------------------------------------- axis passes
@@ -266,8 +280,9 @@ void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double s
 // o and i are the remaining outer / inner indices.  A workgroup owns TK
 // consecutive k1 x TI consecutive i for one (rung, field, o); each pencil
 // lives in one wavefront (L/16 lanes) so the FFT passes need no block barrier.
-// The result is written either in the same layout or transposed to [q][k1],
-// which gives the axis-1 stage contiguous rows.
+// The result is written either in the same layout or in the tiled row layout
+// of the row kernel: k1 is cut into tiles of TKy columns, and tile kt of a
+// field holds rows q = 0..Q-1 of TKy contiguous values, at kt*Q*TKy + q*TKy.
 template <typename T>
 struct AxisArgs {
   const cplx<T>* src;
@@ -292,10 +307,15 @@ struct AxisArgs {
   // 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass)
   int TJ, log2TJ, log2otiles, mode;
   const cplx<T>* tw;
+  // mixed-radix plan and twiddle table of the axis (GEN kernels)
+  FftFactors fac;
+  const cplx<T>* twg;
 };
 
-// Generic form for pencil lengths outside the wave-local range.
-template <typename T, int DIR>
+// Generic form for pencil lengths outside the wave-local range or not powers
+// of two (GEN: mixed-radix FFT with the plan's twiddle table).  Inner tiles
+// may run past I (non-power-of-two axes); those pencils are not stored.
+template <typename T, int DIR, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* buf = (cplx<T>*)smem;
@@ -333,13 +353,19 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
       e = (id / TI) % L;
       kk = id / (TI * L);
     }
-    const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
-    cplx<T> v = src[(int64_t)(k0 + kk) * a.Q + q];
-    if (gt) v = cscale(v, gt[e]);
+    cplx<T> v = {(T)0, (T)0};
+    if (i0 + ii < a.I) {
+      const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+      v = src[(int64_t)(k0 + kk) * a.Q + q];
+      if (gt) v = cscale(v, gt[e]);
+    }
     buf[(kk * TI + ii) * L + e] = v;
   }
   __syncthreads();
-  lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
+  if constexpr (GEN)
+    lds_fft_gen<DIR, T>(buf, a.fac, npen, L, a.twg, tid, nthr);
+  else
+    lds_fft<DIR, T>(buf, a.log2L, npen, L, a.tw + (L - 2), tid, nthr);
   for (int id = tid; id < total; id += nthr) {
     int kk, ii, e;
     if (TI == 1) {
@@ -351,6 +377,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_fft(AxisArgs<T> a) {
       e = (id / TI) % L;
       kk = id / (TI * L);
     }
+    if (i0 + ii >= a.I) continue;
     const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
     dst[(int64_t)(k0 + kk) * a.Q + q] = buf[(kk * TI + ii) * L + e];
   }
@@ -580,7 +607,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[1];
   a.log2L = g.log2n[1];
-  if (a.L < Ef || a.L > 64 * Ef) return false;
+  if (a.L < Ef || a.L > 64 * Ef || (a.L & (a.L - 1))) return false;
   a.src = spec;
   a.dst = dst;
   a.src_rs = 0;
@@ -631,8 +658,8 @@ static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t
 
 // Second inverse pass of a 3-D rung chunk (grid axis 3) on the [k1][j2][j3]
 // layout written by launch_axis_ladder (mode 2): pencils along j3 are
-// contiguous; the output is the transposed [q][k1] layout of the row kernel
-// (q = j2 + n2 * j3).  False when n3 is outside the wave-local range.
+// contiguous; the output is the tiled row layout of the row kernel with rows
+// ordered j3-fastest (q = j3 + n3 * j2).  False when n3 is outside the wave-local range.
 template <typename T>
 bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride,
                        cplx<T>* dst, int64_t dst_rung_stride, int nf, int R, int r0,
@@ -641,7 +668,7 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[2];
   a.log2L = g.log2n[2];
-  if (g.N != 3 || a.L < Ef || a.L > 64 * Ef) return false;
+  if (g.N != 3 || a.L < Ef || a.L > 64 * Ef || (a.L & (a.L - 1))) return false;
   a.src = src;
   a.dst = dst;
   a.src_rs = src_rung_stride;
@@ -730,7 +757,7 @@ template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     hipStream_t s) {
+                     const TI* twg, hipStream_t s) {
   static_assert(sizeof(TI) == sizeof(TO), "axis passes keep the element type");
   AxisArgs<TI> a;
   a.src = src;
@@ -766,7 +793,10 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
   a.transposed = 0;
   a.tw = (const cplx<TI>*)tw;
-  if (a.L >= Ef && a.L <= 64 * Ef) {
+  const bool gen = (a.L & (a.L - 1)) != 0;
+  a.fac = g.fac[axis];
+  a.twg = twg ? (const cplx<TI>*)twg + g.koff[axis] : nullptr;
+  if (!gen && a.L >= Ef && a.L <= 64 * Ef) {
     // wave-local form: TI and TK powers of two, TK * TI <= per_block
     a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, std::min(8, per_block));
     int hp = 1;
@@ -790,16 +820,20 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   }
   a.TK = std::max(1, std::min(g.H1, per_block / a.TI));
   a.ktiles = (g.H1 + a.TK - 1) / a.TK;
-  a.itiles = (int)(a.I / a.TI);
+  a.itiles = (int)((a.I + a.TI - 1) / a.TI);
   const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
   const size_t lds = (size_t)a.TK * a.TI * a.L * sizeof(cplx<TI>);
-  k_axis_fft<TI, DIR><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  if (gen)
+    k_axis_fft<TI, DIR, true><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  else
+    k_axis_fft<TI, DIR, false><<<(unsigned)blocks, kThreads, lds, s>>>(a);
 }
 
 #define PKDE_AXIS_INST(TI, TO, DIR)                                                            \
   template void launch_axis_fft<TI, TO, DIR>(const GridDev&, int, const cplx<TI>*, int64_t, int,  \
                                              cplx<TO>*, int64_t, int, int, int, int, int,        \
-                                             const TI*, const TI*, int, const TI*, hipStream_t);
+                                             const TI*, const TI*, int, const TI*, const TI*,    \
+                                             hipStream_t);
 PKDE_AXIS_INST(double, double, -1)
 PKDE_AXIS_INST(double, double, 1)
 PKDE_AXIS_INST(float, float, 1)
@@ -871,9 +905,11 @@ struct VmrArgs {
   int log2TKy, rows3, log2n2, log2n3;
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
+  FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
+  const cplx<T>* twg;
 };
 
-template <typename T, int PPT>
+template <typename T, int PPT, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* Z = (cplx<T>*)smem;
@@ -908,7 +944,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
       const int slot = rest >> 1;
       const int b = bp + 2 * slot;
       cplx<T>* row = Z + (int64_t)((slot * 2 + ty) * TQ + t) * n1;
-      if (b >= B) continue;
+      if (b >= B || q0 + t >= a.Q) continue;
       const int fa = ty * B + b;
       const T gk = ty ? gv1[k1] : gm1[k1];
       const int64_t o = (int64_t)k1 * a.Q + q0 + t;
@@ -918,7 +954,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
       put_pair(row, n1, k1, A, Bv);
     }
     __syncthreads();
-    lds_fft<1, T>(Z, a.log2n1, npen, n1, a.tw + (n1 - 2), tid, nthr);
+    if constexpr (GEN)
+      lds_fft_gen<1, T>(Z, a.fac, npen, n1, a.twg, tid, nthr);
+    else
+      lds_fft<1, T>(Z, a.log2n1, npen, n1, a.tw + (n1 - 2), tid, nthr);
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int w = tid + u * nthr;
@@ -956,17 +995,33 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
     const int w = tid + u * nthr;
     if (w < npts) {
       const int t = w / n1, j1 = w - t * n1;
+      if (q0 + t >= a.Q) continue;
       const double vv = sc * m2[u] / (double)(B - 1);
       a.vmr[(int64_t)r * a.G + (q0 + t) * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
     }
   }
 }
 
+template <typename T, bool GEN>
+static void ladder_vmr_dispatch(int ppt, unsigned blocks, size_t lds, hipStream_t s,
+                                const VmrArgs<T>& a) {
+  if (ppt <= 1)
+    k_ladder_vmr<T, 1, GEN><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 2)
+    k_ladder_vmr<T, 2, GEN><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 4)
+    k_ladder_vmr<T, 4, GEN><<<blocks, kThreads, lds, s>>>(a);
+  else if (ppt <= 8)
+    k_ladder_vmr<T, 8, GEN><<<blocks, kThreads, lds, s>>>(a);
+  else
+    k_ladder_vmr<T, 16, GEN><<<blocks, kThreads, lds, s>>>(a);
+}
+
 template <typename T>
 void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                        int64_t n_samples, int R, int r0, const T* gM, const T* gV,
-                       const double* cV, const double* vscale, const T* tw, double* vmr,
-                       hipStream_t s) {
+                       const double* cV, const double* vscale, const T* tw, const T* twg,
+                       double* vmr, hipStream_t s) {
   VmrArgs<T> a;
   const int n1 = g.n[0];
   // 2048 points per workgroup where the grid allows it; PPT = points/thread
@@ -1000,32 +1055,32 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.H1 = g.H1;
   a.TQ = TQ;
   a.NBP = NBP;
-  a.ntiles = (int)(g.Q / TQ);
+  a.ntiles = (int)((g.Q + TQ - 1) / TQ);
+  a.fac = g.fac[0];
+  a.twg = twg ? (const cplx<T>*)twg + g.koff[0] : nullptr;
+  const bool gen = (n1 & (n1 - 1)) != 0;
   const unsigned blocks = (unsigned)((int64_t)R * a.ntiles);
   const size_t lds = (size_t)2 * NBP * TQ * n1 * sizeof(cplx<T>);
-  if (ppt <= 1)
-    k_ladder_vmr<T, 1><<<blocks, kThreads, lds, s>>>(a);
-  else if (ppt <= 2)
-    k_ladder_vmr<T, 2><<<blocks, kThreads, lds, s>>>(a);
-  else if (ppt <= 4)
-    k_ladder_vmr<T, 4><<<blocks, kThreads, lds, s>>>(a);
-  else if (ppt <= 8)
-    k_ladder_vmr<T, 8><<<blocks, kThreads, lds, s>>>(a);
+  if (gen)
+    ladder_vmr_dispatch<T, true>(ppt, blocks, lds, s, a);
   else
-    k_ladder_vmr<T, 16><<<blocks, kThreads, lds, s>>>(a);
+    ladder_vmr_dispatch<T, false>(ppt, blocks, lds, s, a);
 }
 
 template void launch_ladder_vmr<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t, int,
                                        int, const float*, const float*, const double*,
-                                       const double*, const float*, double*, hipStream_t);
+                                       const double*, const float*, const float*, double*,
+                                       hipStream_t);
 template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int64_t, int, int64_t,
                                         int, int, const double*, const double*, const double*,
-                                        const double*, const double*, double*, hipStream_t);
+                                        const double*, const double*, const double*, double*,
+                                        hipStream_t);
 
 // ------------------------------------------------- ladder VMR, row per wave
 // Same computation as k_ladder_vmr for grids with Q >= 64 rows and
-// 16 <= n1 <= 512, reading the axis-2/3-transformed fields in the transposed
-// layout [rung][field][q][k1] so every row is contiguous.  Each wavefront owns
+// 16 <= n1 <= 512, reading the axis-2/3-transformed fields in the tiled row
+// layout [rung][field][k1 tile][q][k1 within tile], so each row segment is
+// contiguous.  Each wavefront owns
 // W = 512/n1 rows of one rung, builds the M and V pencils of two bootstraps
 // per row in its own LDS slice, transforms them with wave-local passes and
 // accumulates the FP64 Welford statistics of its 8 points per lane; the
@@ -1322,9 +1377,11 @@ struct MeansArgs {
   double invG;
   int64_t Q;
   int n1, log2n1, H1, TQ, ntiles;
+  FftFactors fac;  // axis-1 plan and twiddle table (GEN kernels)
+  const cplx<T>* twg;
 };
 
-template <typename T>
+template <typename T, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_means_select(MeansArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* Z = (cplx<T>*)smem;
@@ -1342,6 +1399,7 @@ __global__ __launch_bounds__(kThreads) void k_means_select(MeansArgs<T> a) {
   const T* gb = a.gM + (int64_t)rb * a.ksum;
   for (int id = tid; id < H1 * TQ; id += nthr) {
     const int t = id % TQ, k1 = id / TQ;
+    if (q0 + t >= a.Q) continue;
     const int64_t o = (int64_t)k1 * a.Q + q0 + t;
     const cplx<T> A = cscale(sa[o], ga[k1]);
     cplx<T> Bv = {(T)0, (T)0};
@@ -1349,9 +1407,13 @@ __global__ __launch_bounds__(kThreads) void k_means_select(MeansArgs<T> a) {
     put_pair(Z + (int64_t)t * n1, n1, k1, A, Bv);
   }
   __syncthreads();
-  lds_fft<1, T>(Z, a.log2n1, TQ, n1, a.tw + (n1 - 2), tid, nthr);
+  if constexpr (GEN)
+    lds_fft_gen<1, T>(Z, a.fac, TQ, n1, a.twg, tid, nthr);
+  else
+    lds_fft<1, T>(Z, a.log2n1, TQ, n1, a.tw + (n1 - 2), tid, nthr);
   for (int w = tid; w < TQ * n1; w += nthr) {
     const int t = w / n1, j1 = w - t * n1;
+    if (q0 + t >= a.Q) continue;
     const int64_t p = (q0 + t) * n1 + j1;
     const int sp = a.sel[p];
     const cplx<T> z = Z[(int64_t)t * n1 + j1];
@@ -1364,8 +1426,8 @@ __global__ __launch_bounds__(kThreads) void k_means_select(MeansArgs<T> a) {
 
 template <typename T>
 void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int R,
-                         int r0, int S, const T* gM, const T* tw, const int16_t* sel,
-                         double* density, hipStream_t s) {
+                         int r0, int S, const T* gM, const T* tw, const T* twg,
+                         const int16_t* sel, double* density, hipStream_t s) {
   MeansArgs<T> a;
   const int n1 = g.n[0];
   int TQ = 2048 / n1;
@@ -1387,18 +1449,23 @@ void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_
   a.log2n1 = g.log2n[0];
   a.H1 = g.H1;
   a.TQ = TQ;
-  a.ntiles = (int)(g.Q / TQ);
+  a.ntiles = (int)((g.Q + TQ - 1) / TQ);
+  a.fac = g.fac[0];
+  a.twg = twg ? (const cplx<T>*)twg + g.koff[0] : nullptr;
   const unsigned blocks = (unsigned)(((R + 1) / 2) * (int64_t)a.ntiles);
   const size_t lds = (size_t)TQ * n1 * sizeof(cplx<T>);
-  k_means_select<T><<<blocks, kThreads, lds, s>>>(a);
+  if ((n1 & (n1 - 1)) != 0)
+    k_means_select<T, true><<<blocks, kThreads, lds, s>>>(a);
+  else
+    k_means_select<T, false><<<blocks, kThreads, lds, s>>>(a);
 }
 
 template void launch_means_select<float>(const GridDev&, const cplx<float>*, int64_t, int, int, int,
-                                         const float*, const float*, const int16_t*, double*,
-                                         hipStream_t);
+                                         const float*, const float*, const float*, const int16_t*,
+                                         double*, hipStream_t);
 template void launch_means_select<double>(const GridDev&, const cplx<double>*, int64_t, int, int, int,
-                                          const double*, const double*, const int16_t*, double*,
-                                          hipStream_t);
+                                          const double*, const double*, const double*,
+                                          const int16_t*, double*, hipStream_t);
 
 // ---------------------------------------------------------------- Silverman
 // Two-pass sample moments with a fixed reduction tree (deterministic).

@@ -28,6 +28,10 @@ struct GridDev {
   double lo[kMaxDims], hi[kMaxDims], h[kMaxDims];
   int koff[kMaxDims];  // offset of axis i in per-rung Gaussian tables
   int ksum;            // sum of n_i
+  // mixed-radix FFT plans of the axes; pow2 != 0 when every axis is a power
+  // of two (the wave-local kernels and the tiled row layout need that)
+  FftFactors fac[kMaxDims];
+  int pow2;
 };
 
 // ---- launchers (implemented in pkde_kernels.hip) ----
@@ -51,9 +55,11 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
 
 // acc -> forward transform along axis 1 in FP64, Hermitian halves written to
 // specD[fid][k1][q].
+// twg: the plan's per-axis twiddle tables exp(-2 pi i k / n_i) at offsets
+// koff[i] (used for axes that are not powers of two).
 void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double scaleM,
                           double scaleV, int B, cplx<double>* specD, const double* tw64,
-                          hipStream_t s);
+                          const double* twg, hipStream_t s);
 
 // Generic batched pencil FFT along axis `axis` (>= 1, i.e. axis 2 or 3) of
 // [rung][field][k1][q] arrays, in place or out of place, same layout.
@@ -62,7 +68,7 @@ template <typename TI, typename TO, int DIR>
 void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t src_rung_stride,
                      int src_fid0, cplx<TO>* dst, int64_t dst_rung_stride, int dst_fid0, int fid0,
                      int nf, int R, int r0, const TI* gM, const TI* gV, int fsplit, const TI* tw,
-                     hipStream_t s);
+                     const TI* twg, hipStream_t s);
 
 // Ladder form of the first inverse pass along grid axis 2: spectra read once
 // per chunk of R rungs (see k_axis_ladder).  mode: 0 [k1][q], 1 tiled rows
@@ -88,8 +94,8 @@ void launch_convert(const cplx<TI>* src, cplx<TO>* dst, int64_t count, hipStream
 template <typename T>
 void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                        int64_t n_samples, int R, int r0, const T* gM, const T* gV,
-                       const double* cV, const double* vscale, const T* tw, double* vmr,
-                       hipStream_t s);
+                       const double* cV, const double* vscale, const T* tw, const T* twg,
+                       double* vmr, hipStream_t s);
 
 // Row-per-wave form of the same ladder step for Q >= 64 rows and
 // 16 <= n1 <= 512, reading the tiled row layout (see GridDev).
@@ -115,8 +121,8 @@ void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt
 // Full-sample means at each point's selected rung -> density.
 template <typename T>
 void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int R,
-                         int r0, int S, const T* gM, const T* tw, const int16_t* sel,
-                         double* density, hipStream_t s);
+                         int r0, int S, const T* gM, const T* tw, const T* twg,
+                         const int16_t* sel, double* density, hipStream_t s);
 
 // Silverman rule on device (DirectSpace.jl:1102-1111): bw[i] = prefactor * std_i.
 void launch_silverman(int N, const double* x, int64_t n, double prefactor, double* partial,

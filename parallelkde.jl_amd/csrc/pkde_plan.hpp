@@ -33,6 +33,12 @@ struct pkde_ctx {
   std::vector<pkde_batch_lane> lanes;
   unsigned char* stage = nullptr;  // pinned host staging
   size_t stage_bytes = 0;
+  // live plans: destroying the context orphans them (plan->ctx = nullptr), so
+  // a plan destroyed afterwards frees its own memory without touching this.
+  // Guarded by its own lock: the batched form creates and destroys its lane
+  // plans while holding `mu`.
+  std::mutex live_mu;
+  std::vector<pkde_plan*> live;
 };
 
 namespace pkde {
@@ -68,6 +74,8 @@ struct PlanBuffers {
   void* gM = nullptr;
   void* gV = nullptr;
   double* invk = nullptr;  // Welford weights 1 / (b + 1)
+  double* twg64 = nullptr;  // per-axis twiddles exp(-2 pi i k / n_i) at koff[i]
+  float* twg32 = nullptr;
   void* binwork = nullptr;  // workspace of the sorted binning
   uint64_t binwork_bytes = 0;
   int S_alloc = 0;

@@ -14,6 +14,7 @@ fallback in this package (the reference's own CPU path is the Julia code).
 from __future__ import annotations
 
 import math
+import sys
 import threading
 from dataclasses import dataclass, field
 
@@ -83,6 +84,10 @@ class _Context:
         self.lock = threading.Lock()
 
     def __del__(self):
+        # at interpreter shutdown finalizers run in no fixed order; the
+        # process exit releases device memory, so native calls are skipped
+        if sys.is_finalizing():
+            return
         try:
             if self.handle:
                 _lib.load().pkde_context_destroy(self.handle)
@@ -162,9 +167,10 @@ def initialize_grid(ranges, device="rocm", **_):
                 tuple(s[3] for s in specs), tuple(s[2] for s in specs), d)
 
 
-#: Default points per axis of find_grid.  The reference uses 150 (Grids.jl:282);
-#: the :rocm FFTs need powers of two, so the nearest one, 128, is used.
-FIND_GRID_DEFAULT_POINTS = 128
+#: Default points per axis of find_grid, as in the reference (Grids.jl:282; its
+#: docstring says 300, the code uses 150).  150 = 2 x 3 x 5^2 runs on the
+#: mixed-radix FFT path.
+FIND_GRID_DEFAULT_POINTS = 150
 
 
 def find_grid(data, grid_bounds=None, grid_dims=None, grid_steps=None, grid_padding=None,
@@ -272,6 +278,10 @@ class DensityEstimation:
         return get_density(self, normalize=normalize, dx=dx)
 
     def __del__(self):
+        # at interpreter shutdown finalizers run in no fixed order; the
+        # process exit releases device memory, so native calls are skipped
+        if sys.is_finalizing():
+            return
         try:
             for p in self._plans.values():
                 p.close()
@@ -419,6 +429,10 @@ class _Plan:
             self.handle = None
 
     def __del__(self):
+        # at interpreter shutdown finalizers run in no fixed order; the
+        # process exit releases device memory, so native calls are skipped
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -481,6 +495,10 @@ class _RoTPlan:
             self.handle = None
 
     def __del__(self):
+        # at interpreter shutdown finalizers run in no fixed order; the
+        # process exit releases device memory, so native calls are skipped
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -639,6 +657,10 @@ class Server:
             self.handle = None
 
     def __del__(self):
+        # at interpreter shutdown finalizers run in no fixed order; the
+        # process exit releases device memory, so native calls are skipped
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

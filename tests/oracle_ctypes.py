@@ -82,6 +82,7 @@ def lib():
         L.og_dirac_sequences.argtypes = [C.POINTER(OgGrid), _dp, C.c_int64, _i32p, C.c_int32,
                                          _dp, C.c_void_p, C.c_int]
         L.og_fft_nd.argtypes = [C.c_int, C.POINTER(C.c_int64), _dp, C.c_int]
+        L.og_set_fft_order.argtypes = [C.c_int]
         L.og_propagate.argtypes = [C.POINTER(OgGrid), _dp, _dp, _dp, _dp, C.c_void_p, _dp, C.c_void_p]
         L.og_scaled_vmr.argtypes = [C.c_int, C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, C.c_int64, _dp]
         L.og_full_means.argtypes = [C.c_int64, _dp, _dp]
@@ -165,6 +166,25 @@ def fft_nd(arr: np.ndarray, sign: int) -> np.ndarray:
     n = (C.c_int64 * len(shape_cm))(*shape_cm)
     lib().og_fft_nd(len(shape_cm), n, buf, sign)
     return buf.view(np.complex128).reshape(a.shape)
+
+
+def set_fft_order(reversed_: bool):
+    """Radix order of the oracle's FFT (og_set_fft_order)."""
+    lib().og_set_fft_order(1 if reversed_ else 0)
+
+
+def rounding_envelope(g: OgGrid, samples_col, idx_bs, **kw):
+    """The oracle's estimate under its default FFT radix order and under the
+    reversed order, and the distance between the two: how far FP64 rounding
+    alone moves the reference's result on this input.  Returns (default
+    result, reversed result)."""
+    r0 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
+    set_fft_order(True)
+    try:
+        r1 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
+    finally:
+        set_fft_order(False)
+    return r0, r1
 
 
 def dirac_sequences(g: OgGrid, samples_col: np.ndarray, idx_bs: np.ndarray, with_squares=True, threads=1):

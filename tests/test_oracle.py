@@ -60,6 +60,24 @@ def test_fft_brute_force_dft(n):
     assert np.allclose(O.fft_nd(a, -1), W @ a, rtol=0, atol=1e-12 * np.abs(a).sum())
 
 
+@pytest.mark.parametrize("n", [12, 30, 150, 96, 300])
+def test_fft_reversed_radix_order_is_the_same_dft(n):
+    """The reversed radix order (used to measure the rounding envelope) is
+    the same transform: brute-force DFT, within FP64 rounding."""
+    rng = np.random.default_rng(n + 1)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    O.set_fft_order(True)
+    try:
+        f = O.fft_nd(a, -1)
+    finally:
+        O.set_fft_order(False)
+    assert np.allclose(f, W @ a, rtol=0, atol=1e-12 * np.abs(a).sum())
+    if n in (30, 150, 300):  # several distinct factors: a different rounding
+        assert not np.array_equal(f, O.fft_nd(a, -1))
+
+
 def test_fft_round_trip_and_parseval():
     rng = np.random.default_rng(3)
     a = rng.standard_normal((6, 10, 9)) + 1j * rng.standard_normal((6, 10, 9))
