@@ -470,10 +470,11 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // of one field in registers -- each lane holds exactly the inputs of its
 // first Stockham pass -- and loops over the R rungs of the chunk, so the
 // spectra are read once per chunk rather than once per rung.  Per rung:
-// Gaussian factor, first pass from registers, remaining passes through LDS,
-// store of the block's pencils in layout `mode`: 0 [k1][q], 1 [q][k1] rows for
-// the row kernel (2-D), 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D
-// axis-3 pass).  The arithmetic is that of k_axis_wave with DIR = +1.
+// Gaussian factor (staged in LDS one rung ahead), first pass from registers,
+// remaining passes through LDS (twiddles from LDS), store of the block's
+// pencils in layout `mode`: 0 [k1][q], 1 the tiled rows of the row kernel
+// (2-D), 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass).  The
+// arithmetic is that of k_axis_wave with DIR = +1.
 template <typename T, int LOG2L>
 __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -506,6 +507,18 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   const bool act = kk < TKe && p < TK * TJ;
   const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
   const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
+  const int log2np = log2TK + log2TJ;
+  const int total = 1 << (log2np + LOG2L);
+  // LDS after the block's pencils: the twiddles exp(-2 pi i k / L), and the
+  // axis factors of two rungs (the current one and the next, staged one rung
+  // ahead so their loads overlap the current rung's passes)
+  cplx<T>* twl = buf + ((int64_t)PL << log2np);
+  T* gl = (T*)(twl + L);
+  constexpr int GPT = (L + kThreads - 1) / kThreads;
+  for (int k = tid; k < L; k += kThreads) {
+    twl[k] = a.tw[L - 2 + k];
+    gl[k] = gbase[(int64_t)a.r0 * a.ksum + k];
+  }
   cplx<T> x[E];
   if (act) {
     const cplx<T>* col = src + (int64_t)(k0 + kk) * a.Q + (o0 + jj) * L;
@@ -515,10 +528,19 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
       for (int r = 0; r < R1; ++r) x[u * R1 + r] = col[t + u * tpp + r * nb1];
   }
   cplx<T>* pb = buf + (act ? p : 0) * PL;
-  const int log2np = log2TK + log2TJ;
-  const int total = 1 << (log2np + LOG2L);
+  __syncthreads();
   for (int rr = 0; rr < a.R; ++rr) {
-    const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
+    const T* gt = gl + (rr & 1) * L;
+    // next rung's factors (the last rung reloads its own: a valid address)
+    T gn[GPT];
+    {
+      const T* gsrc = gbase + (int64_t)(a.r0 + (rr + 1 < a.R ? rr + 1 : rr)) * a.ksum;
+#pragma unroll
+      for (int i = 0; i < GPT; ++i) {
+        const int k = tid + i * kThreads;
+        gn[i] = gsrc[k < L ? k : 0];
+      }
+    }
     if (act) {
 #pragma unroll
       for (int u = 0; u < E / R1; ++u) {
@@ -532,8 +554,17 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
       }
     }
     wave_sync();
-    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
+    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, twl);
     __syncthreads();
+    {
+      // the other slot was last read before this rung's first barrier
+      T* gw = gl + ((rr + 1) & 1) * L;
+#pragma unroll
+      for (int i = 0; i < GPT; ++i) {
+        const int k = tid + i * kThreads;
+        if (k < L) gw[k] = gn[i];
+      }
+    }
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if (a.mode == 1) {
       // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous
@@ -648,7 +679,9 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.transposed = mode == 1;
   a.tw = (const cplx<T>*)tw;
   const int64_t blocks = (int64_t)nf * (a.O / a.TJ) * a.ktiles;
-  const size_t lds = (size_t)a.TK * a.TJ * pencil_stride(a.L) * sizeof(cplx<T>);
+  // pencils, twiddles, and two rungs of axis factors
+  const size_t lds = ((size_t)a.TK * a.TJ * pencil_stride(a.L) + a.L) * sizeof(cplx<T>) +
+                     2 * (size_t)a.L * sizeof(T);
   return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
 }
 
@@ -1080,11 +1113,12 @@ template void launch_ladder_vmr<double>(const GridDev&, const cplx<double>*, int
 // Same computation as k_ladder_vmr for grids with Q >= 64 rows and
 // 16 <= n1 <= 512, reading the axis-2/3-transformed fields in the tiled row
 // layout [rung][field][k1 tile][q][k1 within tile], so each row segment is
-// contiguous.  Each wavefront owns
-// W = 512/n1 rows of one rung, builds the M and V pencils of two bootstraps
-// per row in its own LDS slice, transforms them with wave-local passes and
-// accumulates the FP64 Welford statistics of its 8 points per lane; the
-// wavefronts of a workgroup never synchronise with each other.
+// contiguous.  Each wavefront owns W = 64 PPT / n1 rows of one rung, builds
+// the M and V pencils of two bootstraps per row in its own LDS slice,
+// transforms them with wave-local passes (twiddles read from LDS) and
+// accumulates the FP64 Welford statistics of its PPT points per lane; after
+// the block stages the twiddles, its wavefronts never synchronise with each
+// other.  Global loads run one bootstrap pair ahead of the arithmetic.
 template <typename T, int E, int PPT, int LOG2N1>
 __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   constexpr int n1 = 1 << LOG2N1;
@@ -1098,6 +1132,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
+  // twiddles exp(-2 pi i k / n1) staged once per block after the waves' rows
+  cplx<T>* twl = (cplx<T>*)smem + (kThreads / 64) * (2 * W * PL);
+  for (int k = threadIdx.x; k < n1; k += kThreads) twl[k] = a.tw[n1 - 2 + k];
+  __syncthreads();
   cplx<T>* Z = (cplx<T>*)smem + wv * (2 * W * PL);
   const int64_t rows_per_rung = a.Q >> LOG2W;
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
@@ -1120,17 +1158,21 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   // Nyquist term k1 = n1/2 of row `lane` for lanes < 2W.  Row index
   // ty * W + w: ty = 0 sums field, ty = 1 squares field.
   T ld_g[PPT];
+  int64_t ld_off[PPT];
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int id = lane + 64 * u;
     const int k1 = id & (H - 1);
-    ld_g[u] = (id >> (LOG2N1 - 1 + LOG2W)) ? gv1[k1] : gm1[k1];
+    const int rw = id >> (LOG2N1 - 1);
+    ld_g[u] = (rw >> LOG2W) ? gv1[k1] : gm1[k1];
+    // squares rows sit B fields after the sums rows
+    ld_off[u] = toff(rw & (W - 1), k1) + ((rw >> LOG2W) ? (int64_t)a.B * a.fsize : 0);
   }
   const bool nyq = lane < 2 * W;
   const int nrow = nyq ? lane : 0;
   const int nty = nrow >> LOG2W, nw = nrow & (W - 1);
   const T ng = nty ? gv1[H] : gm1[H];
-  const int64_t nyq_off = toff(nw, H);
+  const int64_t nyq_off = toff(nw, H) + (nty ? (int64_t)a.B * a.fsize : 0);
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -1139,41 +1181,57 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     m2[u] = 0.0;
   }
   const int B = a.B;
+  // Loads of one bootstrap pair (fields b and b + 1 of the sums and squares
+  // blocks), issued one pair ahead so they are in flight during the current
+  // pair's transform and Welford updates.  Branch-free: a pair without a
+  // second field reloads the first and discards it.
+  cplx<T> pa[PPT], pb2[PPT], pna, pnb;
+  double pi1, pi2;
+  auto issue = [&](int b) {
+    const cplx<T>* f = src + (int64_t)b * a.fsize;
+    const int64_t o2 = b + 1 < B ? a.fsize : 0;
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      pa[u] = f[ld_off[u]];
+      pb2[u] = f[ld_off[u] + o2];
+    }
+    pna = f[nyq_off];
+    pnb = f[nyq_off + o2];
+    pi1 = a.invk[b];
+    pi2 = a.invk[b + 1 < B ? b + 1 : b];
+  };
+  issue(0);
   for (int b = 0; b < B; b += 2) {
     const bool has2 = b + 1 < B;
-    // field b of the sums (ty = 0) or squares (ty = 1) block
-    const cplx<T>* f0 = src + (int64_t)b * a.fsize;
-    const cplx<T>* fS = src + (int64_t)(B + b) * a.fsize;
+    const cplx<T> zero = {(T)0, (T)0};
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int id = lane + 64 * u;
       const int k1 = id & (H - 1);
       const int rw = id >> (LOG2N1 - 1);
-      const int64_t off = toff(rw & (W - 1), k1);
-      const cplx<T>* fb = (rw >> LOG2W) ? fS : f0;
-      const cplx<T> A = cscale(fb[off], ld_g[u]);
-      cplx<T> Bv = {(T)0, (T)0};
-      if (has2) Bv = cscale(fb[a.fsize + off], ld_g[u]);
+      const cplx<T> A = cscale(pa[u], ld_g[u]);
+      const cplx<T> Bv = has2 ? cscale(pb2[u], ld_g[u]) : zero;
       cplx<T>* row = Z + rw * PL;
       row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
       if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
     }
     if (nyq) {
-      const cplx<T>* fb = nty ? fS : f0;
-      const cplx<T> A = cscale(fb[nyq_off], ng);
-      cplx<T> Bv = {(T)0, (T)0};
-      if (has2) Bv = cscale(fb[a.fsize + nyq_off], ng);
+      const cplx<T> A = cscale(pna, ng);
+      const cplx<T> Bv = has2 ? cscale(pnb, ng) : zero;
       Z[nrow * PL + pad8(H)] = {A.re - Bv.im, A.im + Bv.re};
     }
+    const double inv1 = pi1;
+    const double inv2 = has2 ? pi2 : 0.0;
     wave_sync();
 #pragma unroll
     for (int c = 0; c < 2 * W; c += ppc) {
       const int p = c + lane / lp;
       const bool act = p < 2 * W;
-      wave_fft<LOG2N1, 1, T, E>(Z + (act ? p : 0) * PL, lane % lp, act, a.tw + (n1 - 2));
+      wave_fft<LOG2N1, 1, T, E>(Z + (act ? p : 0) * PL, lane % lp, act, twl);
     }
-    const double inv1 = a.invk[b];
-    const double inv2 = has2 ? a.invk[b + 1] : 0.0;
+    // next pair's loads (the last iteration reloads its own pair: valid
+    // addresses, values unused)
+    issue(b + 2 < B ? b + 2 : b);
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int pt = lane + 64 * u;
@@ -1278,7 +1336,7 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   const int64_t waves = (int64_t)R * (g.Q / W);
   const int wpb = kThreads / 64;
   const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
-  const size_t lds = (size_t)wpb * 2 * W * pencil_stride(n1) * sizeof(cplx<T>);
+  const size_t lds = ((size_t)wpb * 2 * W * pencil_stride(n1) + n1) * sizeof(cplx<T>);
   if (ppt == 4)
     rows_dispatch<T, E, 4>(g.log2n[0], blocks, lds, s, a);
   else
@@ -1313,8 +1371,7 @@ __global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, cons
     unsigned cl = 0, co = 0;
     bool low = false;
     int ch = S;
-    for (int r = 0; r < S; ++r) {
-      const double f = vmr[(int64_t)r * G + p];
+    auto step = [&](const double f, const int r) {
       const double dl = dlogt[r];
       const double d1 = log(fabs(__ddiv_rn(__dsub_rn(f, f1), __dmul_rn(2.0, dl))));
       const double d2 = log(fabs(__ddiv_rn(__dadd_rn(__dsub_rn(f, __dmul_rn(2.0, f1)), f2),
@@ -1348,6 +1405,27 @@ __global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, cons
       if (more_stable) ch = r;
       f2 = f1;
       f1 = f;
+    };
+    // the table is read RB rungs ahead of the state machine, so a thread keeps
+    // RB loads in flight (one thread per point leaves few waves per SIMD)
+    constexpr int RB = 16;
+    double nxt[RB];
+    auto fetch = [&](int r0) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const int r = r0 + i < S ? r0 + i : S - 1;
+        nxt[i] = vmr[(int64_t)r * G + p];
+      }
+    };
+    fetch(0);
+    for (int r0 = 0; r0 < S; r0 += RB) {
+      double cur[RB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) cur[i] = nxt[i];
+      if (r0 + RB < S) fetch(r0 + RB);
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+        if (r0 + i < S) step(cur[i], r0 + i);
     }
     chosen[p] = (int16_t)ch;
     sel[p] = (int16_t)(ch == S ? S - 1 : ch);
