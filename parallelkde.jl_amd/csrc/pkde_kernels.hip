@@ -470,11 +470,11 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // of one field in registers -- each lane holds exactly the inputs of its
 // first Stockham pass -- and loops over the R rungs of the chunk, so the
 // spectra are read once per chunk rather than once per rung.  Per rung:
-// Gaussian factor (staged in LDS one rung ahead), first pass from registers,
-// remaining passes through LDS (twiddles from LDS), store of the block's
-// pencils in layout `mode`: 0 [k1][q], 1 the tiled rows of the row kernel
-// (2-D), 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass).  The
-// arithmetic is that of k_axis_wave with DIR = +1.
+// Gaussian factor, first pass from registers, remaining passes through LDS,
+// store of the block's pencils in layout `mode`: 0 [k1][q], 1 the tiled rows
+// of the row kernel (2-D; the block's tile is one contiguous run), 2
+// [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass).  The arithmetic
+// is that of k_axis_wave with DIR = +1.
 template <typename T, int LOG2L>
 __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -507,18 +507,6 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   const bool act = kk < TKe && p < TK * TJ;
   const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
   const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
-  const int log2np = log2TK + log2TJ;
-  const int total = 1 << (log2np + LOG2L);
-  // LDS after the block's pencils: the twiddles exp(-2 pi i k / L), and the
-  // axis factors of two rungs (the current one and the next, staged one rung
-  // ahead so their loads overlap the current rung's passes)
-  cplx<T>* twl = buf + ((int64_t)PL << log2np);
-  T* gl = (T*)(twl + L);
-  constexpr int GPT = (L + kThreads - 1) / kThreads;
-  for (int k = tid; k < L; k += kThreads) {
-    twl[k] = a.tw[L - 2 + k];
-    gl[k] = gbase[(int64_t)a.r0 * a.ksum + k];
-  }
   cplx<T> x[E];
   if (act) {
     const cplx<T>* col = src + (int64_t)(k0 + kk) * a.Q + (o0 + jj) * L;
@@ -528,19 +516,10 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
       for (int r = 0; r < R1; ++r) x[u * R1 + r] = col[t + u * tpp + r * nb1];
   }
   cplx<T>* pb = buf + (act ? p : 0) * PL;
-  __syncthreads();
+  const int log2np = log2TK + log2TJ;
+  const int total = 1 << (log2np + LOG2L);
   for (int rr = 0; rr < a.R; ++rr) {
-    const T* gt = gl + (rr & 1) * L;
-    // next rung's factors (the last rung reloads its own: a valid address)
-    T gn[GPT];
-    {
-      const T* gsrc = gbase + (int64_t)(a.r0 + (rr + 1 < a.R ? rr + 1 : rr)) * a.ksum;
-#pragma unroll
-      for (int i = 0; i < GPT; ++i) {
-        const int k = tid + i * kThreads;
-        gn[i] = gsrc[k < L ? k : 0];
-      }
-    }
+    const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
     if (act) {
 #pragma unroll
       for (int u = 0; u < E / R1; ++u) {
@@ -554,17 +533,8 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
       }
     }
     wave_sync();
-    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, twl);
+    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
     __syncthreads();
-    {
-      // the other slot was last read before this rung's first barrier
-      T* gw = gl + ((rr + 1) & 1) * L;
-#pragma unroll
-      for (int i = 0; i < GPT; ++i) {
-        const int k = tid + i * kThreads;
-        if (k < L) gw[k] = gn[i];
-      }
-    }
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if (a.mode == 1) {
       // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous
@@ -679,9 +649,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.transposed = mode == 1;
   a.tw = (const cplx<T>*)tw;
   const int64_t blocks = (int64_t)nf * (a.O / a.TJ) * a.ktiles;
-  // pencils, twiddles, and two rungs of axis factors
-  const size_t lds = ((size_t)a.TK * a.TJ * pencil_stride(a.L) + a.L) * sizeof(cplx<T>) +
-                     2 * (size_t)a.L * sizeof(T);
+  const size_t lds = (size_t)a.TK * a.TJ * pencil_stride(a.L) * sizeof(cplx<T>);
   return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
 }
 
