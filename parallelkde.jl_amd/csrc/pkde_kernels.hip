@@ -283,6 +283,15 @@ void launch_forward_axis1(const GridDev& g, const void* acc, int fixed, double s
 // The result is written either in the same layout or in the tiled row layout
 // of the row kernel: k1 is cut into tiles of TKy columns, and tile kt of a
 // field holds rows q = 0..Q-1 of TKy contiguous values, at kt*Q*TKy + q*TKy.
+// Columns of a tile row to store when TKe of its TK columns hold data: up to
+// the end of the 128-byte line, so every store covers whole lines.
+template <typename T>
+__device__ __forceinline__ int tile_cols_written(int TKe, int TK) {
+  constexpr int CL = 128 / (int)sizeof(cplx<T>);
+  const int w = (TKe + CL - 1) / CL * CL;
+  return w < TK ? w : TK;
+}
+
 template <typename T>
 struct AxisArgs {
   const cplx<T>* src;
@@ -410,6 +419,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   const int TI = 1 << log2TI, TK = 1 << log2TK;
   const int k0 = (int)kt * TK;
   const int TKe = (a.H1 - k0) < TK ? (a.H1 - k0) : TK;
+  const int TKw = tile_cols_written<T>(TKe, TK);
   const int64_t i0 = (int64_t)it * TI;
   const T* gt = nullptr;
   if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
@@ -428,6 +438,9 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
       cplx<T> v = src[(int64_t)(k0 + kk) * a.Q + q];
       if (gt) v = cscale(v, gt[e]);
       buf[(kk * TI + ii) * PL + pad8(e)] = v;
+    } else if (a.transposed && kk < TKw) {
+      // columns past H1 up to the end of the 128-byte line: zeros
+      buf[(kk * TI + ii) * PL + pad8(e)] = {(T)0, (T)0};
     }
   }
   __syncthreads();
@@ -444,10 +457,11 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   if (a.transposed) {
     // tiled rows (TI == 1): this block's tile is kt, its rows o * oO + e * oE
     cplx<T>* tile = dst + (int64_t)kt * a.Q * TK;
+    // whole 128-byte lines: the last tile's row is padded with zero columns
     for (int id = tid; id < total; id += kThreads) {
       const int kk = id & (TK - 1);
       const int e = id >> log2TK;
-      if (kk < TKe) {
+      if (kk < TKw) {
         const int64_t q = (int64_t)o * a.oO + (int64_t)e * a.oE;
         tile[q * TK + kk] = buf[kk * PL + pad8(e)];
       }
@@ -518,6 +532,13 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   cplx<T>* pb = buf + (act ? p : 0) * PL;
   const int log2np = log2TK + log2TJ;
   const int total = 1 << (log2np + LOG2L);
+  const int TKw = tile_cols_written<T>(TKe, TK);
+  if (a.mode == 1 && TKe < TKw) {
+    // columns past H1 up to the end of the 128-byte line: zero pencils,
+    // stored with the tile (no pass writes them: their lanes are inactive)
+    for (int id = tid; id < ((TKw - TKe) << LOG2L); id += kThreads)
+      buf[(TKe + (id >> LOG2L)) * PL + pad8(id & (L - 1))] = {(T)0, (T)0};
+  }
   for (int rr = 0; rr < a.R; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
     if (act) {
@@ -537,12 +558,15 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if (a.mode == 1) {
-      // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous
+      // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous;
+      // rows are written in whole 128-byte lines (zero columns pad the last
+      // tile), since stores that cover only part of a line cost far more
+      // HBM time
       cplx<T>* tile = dst + (int64_t)kt * a.Q * TK;
       for (int id = tid; id < total; id += kThreads) {
         const int k = id & (TK - 1);
         const int e = id >> log2TK;
-        if (k < TKe) tile[(o0 * L + e) * TK + k] = buf[k * PL + pad8(e)];
+        if (k < TKw) tile[(o0 * L + e) * TK + k] = buf[k * PL + pad8(e)];
       }
     } else if (a.mode == 2) {
       // [k1][j2][j3]: consecutive j3 (outer index) for one (k1, j2)
