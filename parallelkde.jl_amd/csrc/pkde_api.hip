@@ -63,6 +63,20 @@ static bool factorize(int L, FftFactors& f) {
   return n == 1;
 }
 
+// Radices 8, 4, 2, 3, 5 for the wave-local mixed-radix FFT; np = 0 when L
+// has another prime factor.
+static void factorize_wave(int L, FftFactors& f) {
+  f.L = L;
+  f.np = 0;
+  int n = L;
+  while (n % 8 == 0 && f.np < 16) f.rad[f.np++] = 8, n /= 8;
+  if (n % 4 == 0) f.rad[f.np++] = 4, n /= 4;
+  else if (n % 2 == 0) f.rad[f.np++] = 2, n /= 2;
+  for (int p : {3, 5})
+    while (n % p == 0 && f.np < 16) f.rad[f.np++] = p, n /= p;
+  if (n != 1) f.np = 0;
+}
+
 static int ilog2(int64_t v) {
   int l = 0;
   while ((1ll << l) < v) ++l;
@@ -297,6 +311,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const bool rows = g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
                       (N == 2 || wave_ok(g.n[2]));
     const bool rows3 = N == 3 && rows;
+    const bool rows_gen = sizeof(T) == 8 && ladder_gen_rows_ok(g);
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
     const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
@@ -307,6 +322,15 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       // first pass (grid axis 2): spectra read once per chunk.  3-D with the
       // row kernel: the first pass leaves axis 3 contiguous ([k1][j2][j3]),
       // so the axis-3 pass reads whole pencils and writes [q][k1] rows.
+      if constexpr (sizeof(T) == 8) {
+        if (rows_gen) {
+          if (!launch_axis_gen_rows(g, spec, Y, per_rung, 2 * B, R, r0, gM, gV, B, twg, s) ||
+              !launch_ladder_rows_gen(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, twg,
+                                      d.invk, d.vmr, s))
+            return fail(PKDE_DEVICE_ERROR, "mixed-radix ladder layout mismatch");
+          continue;
+        }
+      }
       cplx<T>* Y1 = N == 2 ? Y : Y2;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
@@ -552,6 +576,7 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
                   "grid axis sizes must be in [2, 4096] with no prime factor above 13");
     }
     if (!is_pow2(n_per_axis[i])) g.pow2 = 0;
+    factorize_wave((int)n_per_axis[i], g.facw[i]);
     if (!(h[i] > 0) || !(hi[i] > lo[i]) || !isfinite(lo[i]) || !isfinite(hi[i])) {
       delete pl;
       return fail(PKDE_INVALID_ARGUMENT, "invalid grid bounds or spacing");
@@ -577,6 +602,9 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     int hp = 1;
     while (hp < g.H1) hp <<= 1;
     g.TKy = ndims >= 2 ? std::min(hp, std::max(1, 256 * E / L)) : 1;
+    // axes that are not powers of two: 8-column tiles (128-byte FP64 rows) of
+    // the mixed-radix 2-D ladder
+    if (!g.pow2 && ndims >= 2) g.TKy = 8;
     g.log2TKy = ilog2(g.TKy);
     g.ktY = (g.H1 + g.TKy - 1) / g.TKy;
     g.fsY = (int64_t)g.ktY * g.Q * g.TKy;

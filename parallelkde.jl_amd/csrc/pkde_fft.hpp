@@ -411,4 +411,132 @@ __device__ __forceinline__ void wave_fft(cplx<T>* pb, int t, bool active,
   wave_fft_from<LOG2L, 0, E, DIR, T>(pb, t, active, tw);
 }
 
+// ------------------------------------------- wave-local mixed-radix passes
+// Radix-3 and radix-5 butterflies with the roots taken from the length-L
+// table (w_m = tw[m L / R], conjugated for the inverse), in the symmetric
+// form: y_s and y_{R-s} share their real combinations.
+template <int DIR, typename T>
+__device__ __forceinline__ void dft3(cplx<T>* v, const cplx<T>* __restrict__ tw, int LoverR) {
+  cplx<T> w = tw[LoverR];
+  if (DIR > 0) w.im = -w.im;
+  const cplx<T> t1 = cadd(v[1], v[2]);
+  const cplx<T> t2 = csub(v[1], v[2]);
+  const cplx<T> m = {v[0].re + w.re * t1.re, v[0].im + w.re * t1.im};
+  const cplx<T> r = {-w.im * t2.im, w.im * t2.re};  // i * Im(w) * (v1 - v2)
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft5(cplx<T>* v, const cplx<T>* __restrict__ tw, int LoverR) {
+  cplx<T> w1 = tw[LoverR], w2 = tw[2 * LoverR];
+  if (DIR > 0) {
+    w1.im = -w1.im;
+    w2.im = -w2.im;
+  }
+  const cplx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  const cplx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  const cplx<T> m1 = {v[0].re + w1.re * a1.re + w2.re * a2.re, v[0].im + w1.re * a1.im + w2.re * a2.im};
+  const cplx<T> m2 = {v[0].re + w2.re * a1.re + w1.re * a2.re, v[0].im + w2.re * a1.im + w1.re * a2.im};
+  // i * (s1 b1 + s2 b2) and i * (s2 b1 - s1 b2)
+  const cplx<T> n1 = {-(w1.im * b1.im + w2.im * b2.im), w1.im * b1.re + w2.im * b2.re};
+  const cplx<T> n2 = {-(w2.im * b1.im - w1.im * b2.im), w2.im * b1.re - w1.im * b2.re};
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(m1, n1);
+  v[4] = csub(m1, n1);
+  v[2] = cadd(m2, n2);
+  v[3] = csub(m2, n2);
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft_gen(cplx<T>* v, const cplx<T>* __restrict__ tw, int L) {
+  if constexpr (R == 2 || R == 4 || R == 8 || R == 16)
+    dft_reg<R, DIR, T>(v);
+  else if constexpr (R == 3)
+    dft3<DIR, T>(v, tw, L / 3);
+  else if constexpr (R == 5)
+    dft5<DIR, T>(v, tw, L / 5);
+  else
+    dft_prime<R, DIR, T>(v, tw, L / R);
+}
+
+// One Stockham pass of radix R over a pencil of length L owned by LP lanes
+// of one wavefront (padded layout), lane t doing butterflies t, t + LP, ...
+// (at most NB of them).  All inputs are read before the outputs are written,
+// so the pass is in place.  tw[k] = exp(-2 pi i k / L), k < L.
+template <int R, int NB, int LP, int DIR, typename T>
+__device__ __forceinline__ void wave_pass_gen(cplx<T>* pb, int t, bool active, int L, int Ns,
+                                              const cplx<T>* __restrict__ tw) {
+  const int nb = L / R;
+  const int twstep = L / (Ns * R);
+  const double invNs = 1.0 / (double)Ns;  // b / Ns for b < 2^20 exactly
+  cplx<T> v[NB][R];
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = t + i * LP;
+      if (b < nb) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = pb[pad8(b + r * nb)];
+      }
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = t + i * LP;
+      if (b < nb) {
+        const int q = (int)(((double)b + 0.5) * invNs);
+        const int jm = b - q * Ns;
+        cplx<T>* w = v[i];
+        if (jm != 0) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            cplx<T> f = tw[jm * r * twstep];
+            if (DIR > 0) f.im = -f.im;
+            w[r] = cmul(w[r], f);
+          }
+        }
+        dft_gen<R, DIR, T>(w, tw, L);
+        const int out0 = q * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) pb[pad8(out0 + r * Ns)] = w[r];
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Butterflies per lane the wave-local mixed-radix passes provide for radix R
+// (at most 8 values per lane); a length fits when ceil(L / R / LP) does not
+// exceed it for every pass.
+__host__ __device__ constexpr int wave_gen_nb(int R) {
+  return R == 2 ? 4 : R == 3 ? 2 : R == 4 ? 2 : R == 5 ? 1 : R == 8 ? 1 : 0;
+}
+
+// Wave-local FFT of a length f.L = 2^a 3^b 5^c (natural order in and out,
+// unnormalised), the pencil owned by LP lanes of one wavefront, with the
+// radices of f (each 2, 3, 4, 5 or 8; uniform per wave, so the switch does
+// not diverge).  Larger radices are left to the block-level form: the
+// registers of the largest case bound the kernel's occupancy.
+template <int DIR, typename T, int LP>
+__device__ __forceinline__ void wave_fft_gen(cplx<T>* pb, int t, bool active, const FftFactors& f,
+                                             const cplx<T>* __restrict__ tw) {
+  const int L = f.L;
+  int Ns = 1;
+  for (int p = 0; p < f.np; ++p) {
+    const int R = f.rad[p];
+    switch (R) {
+      case 2: wave_pass_gen<2, wave_gen_nb(2), LP, DIR, T>(pb, t, active, L, Ns, tw); break;
+      case 3: wave_pass_gen<3, wave_gen_nb(3), LP, DIR, T>(pb, t, active, L, Ns, tw); break;
+      case 4: wave_pass_gen<4, wave_gen_nb(4), LP, DIR, T>(pb, t, active, L, Ns, tw); break;
+      case 5: wave_pass_gen<5, wave_gen_nb(5), LP, DIR, T>(pb, t, active, L, Ns, tw); break;
+      default: wave_pass_gen<8, wave_gen_nb(8), LP, DIR, T>(pb, t, active, L, Ns, tw); break;
+    }
+    Ns *= R;
+  }
+}
+
 }  // namespace pkde

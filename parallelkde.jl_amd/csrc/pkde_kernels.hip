@@ -1344,6 +1344,277 @@ template void launch_ladder_rows<double>(const GridDev&, const cplx<double>*, in
                                          const double*, const double*, const double*,
                                          const double*, double*, hipStream_t);
 
+// ------------------------------------------------- mixed-radix 2-D ladder
+// The 2-D ladder for axis lengths that are not powers of two (FP64), in the
+// same two steps as the power-of-two path: a per-rung axis-2 pass writing the
+// tiled row layout (tiles of 8 columns, whole 128-byte lines), then one row
+// per wavefront for the axis-1 transform, VMR and Welford.  Both use the
+// wave-local mixed-radix FFT, one pencil at a time over the 64 lanes of a
+// wavefront, with the plan's twiddle tables staged in LDS.
+constexpr int kGenTK = 8;  // tile width of the tiled row layout (FP64)
+
+__host__ __device__ constexpr int gen_pencil_stride(int L) { return L + L / 8 + 1; }
+
+// Whether every pass of the plan f fits the wave-local mixed-radix FFT with
+// 64 lanes per pencil.
+bool wave_gen_fits(const FftFactors& f) {
+  if (f.L < 2 || f.L > 512) return false;
+  int Ns = 1;
+  for (int p = 0; p < f.np; ++p) {
+    const int R = f.rad[p];
+    const int per_lane = (f.L / R + 63) / 64;
+    if (wave_gen_nb(R) == 0 || per_lane > wave_gen_nb(R)) return false;
+    Ns *= R;
+  }
+  return Ns == f.L;
+}
+
+// One block per (rung, field, 8-column tile): load the tile's spectrum
+// columns along grid axis 2 with the rung's Gaussian factor, transform, store
+// the tile rows.  Columns past H1 up to the tile width are zero.
+__global__ __launch_bounds__(kThreads) void k_axis_gen_rows(AxisArgs<double> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* buf = (cplx<double>*)smem;
+  const int L = a.L;
+  const int PL = gen_pencil_stride(L);
+  cplx<double>* twl = buf + kGenTK * PL;
+  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
+  uint32_t bid = blockIdx.x;
+  const int kt = (int)(bid % (uint32_t)a.ktiles);
+  bid /= (uint32_t)a.ktiles;
+  const int f = a.fid0 + (int)(bid % (uint32_t)a.nf);
+  const int rr = (int)(bid / (uint32_t)a.nf);
+  const int k0 = kt * kGenTK;
+  const int TKe = (a.H1 - k0) < kGenTK ? (a.H1 - k0) : kGenTK;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  // wave wv owns columns 2 wv and 2 wv + 1 of the tile
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int kk = 2 * wv + c;
+    const bool act = kk < TKe;
+    const cplx<double>* col =
+        a.src + (int64_t)(f - a.src_fid0) * a.fsize + (int64_t)(k0 + (act ? kk : 0)) * a.Q;
+    cplx<double>* pb = buf + kk * PL;
+    for (int e = lane; e < L; e += 64)
+      pb[pad8(e)] = act ? cscale(col[e], gt[e]) : cplx<double>{0.0, 0.0};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int kk = 2 * wv + c;
+    if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
+  }
+  __syncthreads();
+  cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
+                       (int64_t)kt * a.Q * kGenTK;
+  for (int id = tid; id < kGenTK * L; id += kThreads) {
+    const int k = id & (kGenTK - 1);
+    const int e = id >> 3;
+    tile[(int64_t)e * kGenTK + k] = buf[k * PL + pad8(e)];
+  }
+}
+
+bool ladder_gen_rows_ok(const GridDev& g) {
+  return g.N == 2 && !g.pow2 && g.TKy == kGenTK && (g.n[0] + 63) / 64 <= 8 &&
+         wave_gen_fits(g.facw[0]) && wave_gen_fits(g.facw[1]);
+}
+
+bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<double>* dst,
+                          int64_t dst_rung_stride, int nf, int R, int r0, const double* gM,
+                          const double* gV, int fsplit, const double* twg, hipStream_t s) {
+  if (!ladder_gen_rows_ok(g)) return false;
+  AxisArgs<double> a{};
+  a.src = spec;
+  a.dst = dst;
+  a.dst_rs = dst_rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.dst_fsize = g.fsY;
+  a.nf = nf;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.fsplit = fsplit;
+  a.ksum = g.ksum;
+  a.goff = g.koff[1];
+  a.L = g.n[1];
+  a.Q = g.Q;
+  a.H1 = g.H1;
+  a.ktiles = g.ktY;
+  a.fac = g.facw[1];
+  a.twg = (const cplx<double>*)twg + g.koff[1];
+  const int64_t blocks = (int64_t)R * nf * g.ktY;
+  const size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
+  k_axis_gen_rows<<<(unsigned)blocks, kThreads, lds, s>>>(a);
+  return true;
+}
+
+// One wavefront per (rung, row q): the row's sums and squares fields of a
+// bootstrap pair form two complex pencils, transformed one after the other;
+// then VMR and Welford over b in order for its PPT points per lane.  Loads
+// run one pair ahead.  Same arithmetic as k_ladder_rows.
+template <int PPT>
+__global__ __launch_bounds__(kThreads) void k_ladder_rows_gen(VmrArgs<double> a) {
+  constexpr int IT = PPT + 1;  // half-spectrum items per lane (2 H1 <= 64 PPT + 2)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = a.n1, H1 = a.H1;
+  const int PL = gen_pencil_stride(n1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  cplx<double>* twl = (cplx<double>*)smem + (kThreads / 64) * 2 * PL;
+  for (int k = threadIdx.x; k < n1; k += kThreads) twl[k] = a.twg[k];
+  __syncthreads();
+  cplx<double>* Z = (cplx<double>*)smem + wv * 2 * PL;
+  const int64_t gw = (int64_t)blockIdx.x * (kThreads / 64) + wv;
+  if (gw >= (int64_t)a.R * a.Q) return;
+  const int rr = (int)(gw / a.Q);
+  const int64_t q = gw - (int64_t)rr * a.Q;
+  const int r = a.r0 + rr;
+  const double* gm1 = a.gM + (int64_t)r * a.ksum;
+  const double* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  // item u: k1 = id mod H1 of the sums (ty 0) or squares (ty 1) row
+  int it_k1[IT];
+  bool it_sq[IT], it_ok[IT];
+  double it_g[IT];
+  int64_t it_off[IT];
+#pragma unroll
+  for (int u = 0; u < IT; ++u) {
+    const int id = lane + 64 * u;
+    it_ok[u] = id < 2 * H1;
+    it_sq[u] = id >= H1;
+    const int k1 = it_ok[u] ? (it_sq[u] ? id - H1 : id) : 0;
+    it_k1[u] = k1;
+    it_g[u] = it_sq[u] ? gv1[k1] : gm1[k1];
+    it_off[u] = (int64_t)(k1 >> 3) * a.tstride + q * kGenTK + (k1 & 7) +
+                (it_sq[u] ? (int64_t)a.B * a.fsize : 0);
+  }
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+  const int B = a.B;
+  cplx<double> pa[IT], pb2[IT];
+  double pi1, pi2;
+  auto issue = [&](int b) {
+    const cplx<double>* fb = src + (int64_t)b * a.fsize;
+    const int64_t o2 = b + 1 < B ? a.fsize : 0;
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      pa[u] = fb[it_off[u]];
+      pb2[u] = fb[it_off[u] + o2];
+    }
+    pi1 = a.invk[b];
+    pi2 = a.invk[b + 1 < B ? b + 1 : b];
+  };
+  issue(0);
+  for (int b = 0; b < B; b += 2) {
+    const bool has2 = b + 1 < B;
+    const cplx<double> zero = {0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      if (it_ok[u]) {
+        const int k1 = it_k1[u];
+        const cplx<double> A = cscale(pa[u], it_g[u]);
+        const cplx<double> Bv = has2 ? cscale(pb2[u], it_g[u]) : zero;
+        cplx<double>* row = Z + (it_sq[u] ? PL : 0);
+        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        if (k1 > 0 && 2 * k1 < n1) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      }
+    }
+    const double inv1 = pi1;
+    const double inv2 = has2 ? pi2 : 0.0;
+    wave_sync();
+    wave_fft_gen<1, double, 64>(Z, lane, true, a.fac, twl);
+    wave_fft_gen<1, double, 64>(Z + PL, lane, true, a.fac, twl);
+    issue(b + 2 < B ? b + 2 : b);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int j1 = lane + 64 * u;
+      if (j1 < n1) {
+        const cplx<double> zm = Z[pad8(j1)];
+        const cplx<double> zv = Z[PL + pad8(j1)];
+        {
+          const double am = fabs(zm.re) * a.invG;
+          const double av = fabs(zv.re) * vfac;
+          const double xv = vmr_ratio(av, am, a.invn);
+          const double d = xv - mean[u];
+          mean[u] += d * inv1;
+          m2[u] += d * (xv - mean[u]);
+        }
+        if (has2) {
+          const double am = fabs(zm.im) * a.invG;
+          const double av = fabs(zv.im) * vfac;
+          const double xv = vmr_ratio(av, am, a.invn);
+          const double d = xv - mean[u];
+          mean[u] += d * inv2;
+          m2[u] += d * (xv - mean[u]);
+        }
+      }
+    }
+    wave_sync();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int j1 = lane + 64 * u;
+    if (j1 < n1) {
+      const double vv = sc * m2[u] / (double)(B - 1);
+      a.vmr[(int64_t)r * a.G + q * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+    }
+  }
+}
+
+bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride,
+                            int B, int64_t n_samples, int R, int r0, const double* gM,
+                            const double* gV, const double* cV, const double* vscale,
+                            const double* twg, const double* invk, double* vmr, hipStream_t s) {
+  const int n1 = g.n[0];
+  const int ppt = (n1 + 63) / 64;
+  if (!ladder_gen_rows_ok(g)) return false;
+  VmrArgs<double> a{};
+  a.src = src;
+  a.src_rs = src_rung_stride;
+  a.fsize = g.fsY;
+  a.tstride = (int64_t)g.Q * kGenTK;
+  a.B = B;
+  a.invn = 1.0 / (double)n_samples;
+  a.invG = 1.0 / (double)g.G;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.ksum = g.ksum;
+  a.cV = cV;
+  a.vscale = vscale;
+  a.invk = invk;
+  a.vmr = vmr;
+  a.G = g.G;
+  a.Q = g.Q;
+  a.n1 = n1;
+  a.H1 = g.H1;
+  a.fac = g.facw[0];
+  a.twg = (const cplx<double>*)twg + g.koff[0];
+  const int64_t waves = (int64_t)R * g.Q;
+  const int wpb = kThreads / 64;
+  const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
+  const size_t lds = ((size_t)wpb * 2 * gen_pencil_stride(n1) + n1) * sizeof(cplx<double>);
+  switch (ppt) {
+    case 1: k_ladder_rows_gen<1><<<blocks, kThreads, lds, s>>>(a); break;
+    case 2: k_ladder_rows_gen<2><<<blocks, kThreads, lds, s>>>(a); break;
+    case 3: k_ladder_rows_gen<3><<<blocks, kThreads, lds, s>>>(a); break;
+    case 4: k_ladder_rows_gen<4><<<blocks, kThreads, lds, s>>>(a); break;
+    case 5: k_ladder_rows_gen<5><<<blocks, kThreads, lds, s>>>(a); break;
+    case 6: k_ladder_rows_gen<6><<<blocks, kThreads, lds, s>>>(a); break;
+    case 7: k_ladder_rows_gen<7><<<blocks, kThreads, lds, s>>>(a); break;
+    default: k_ladder_rows_gen<8><<<blocks, kThreads, lds, s>>>(a); break;
+  }
+  return true;
+}
+
 // -------------------------------------------------------------- convergence
 // find_stability (DirectSpace.jl:975-1031) applied rung after rung with the
 // history rotation of update_density_state_vmr! (GradeProEstimator.jl:617-625);

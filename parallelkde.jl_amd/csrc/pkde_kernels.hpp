@@ -31,6 +31,7 @@ struct GridDev {
   // mixed-radix FFT plans of the axes; pow2 != 0 when every axis is a power
   // of two (the wave-local kernels and the tiled row layout need that)
   FftFactors fac[kMaxDims];
+  FftFactors facw[kMaxDims];  // radices 8, 4, 2, 3, 5 (np = 0: other primes)
   int pow2;
 };
 
@@ -74,6 +75,20 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
 // per chunk of R rungs (see k_axis_ladder).  mode: 0 [k1][q], 1 tiled rows
 // (row-kernel layout, 2-D), 2 [k1][j2][j3] (input of launch_axis3_rows).  False if
 // the axis length is outside the wave-local range.
+// 2-D ladder for axis lengths that are not powers of two (FP64): a per-rung
+// axis-2 pass into the tiled row layout (8-column tiles) and the row kernel on
+// the wave-local mixed-radix FFT.  ladder_gen_rows_ok: the grid qualifies
+// (N == 2, some axis not a power of two, both axes <= 512 with radices the
+// wave-local FFT covers, GridDev::TKy == 8).
+bool ladder_gen_rows_ok(const GridDev& g);
+bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<double>* dst,
+                          int64_t dst_rung_stride, int nf, int R, int r0, const double* gM,
+                          const double* gV, int fsplit, const double* twg, hipStream_t s);
+bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride,
+                            int B, int64_t n_samples, int R, int r0, const double* gM,
+                            const double* gV, const double* cV, const double* vscale,
+                            const double* twg, const double* invk, double* vmr, hipStream_t s);
+
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
