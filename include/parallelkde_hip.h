@@ -93,6 +93,17 @@ void pkde_context_destroy(pkde_ctx *ctx);
 /* Free device memory in bytes (Devices.jl:103-108, get_available_memory). */
 int pkde_device_free_memory(pkde_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
+/* Grid bounds from device-resident samples: the device side of find_grid
+ * (Grids.jl:252-300).  Per axis, the extrema of the samples (Grids.jl:271-273)
+ * widened on both sides by padding[i], or by 0.1 * |max - min| when padding
+ * is NULL (Grids.jl:274-281).  d_samples is ndims x n in device memory,
+ * coordinate fastest.  The caller picks the point counts (the reference's
+ * default is 150 per axis, Grids.jl:284) and the spacing (hi - lo) / (n - 1)
+ * of range(lo, stop=hi, length=n).  PKDE_INVALID_ARGUMENT for non-finite
+ * samples. */
+int pkde_find_grid(pkde_ctx *ctx, int ndims, const double *d_samples, int64_t n,
+                   const double *padding, double *lo_out, double *hi_out);
+
 /* A plan binds a grid (Grid: Grids.jl:67-87 -- per-axis lo, hi, spacing and
  * point count) and the estimator keywords.  Axis sizes must lie in [2, 4096]
  * and have no prime factor above 13 (the radices of the mixed-radix FFT;

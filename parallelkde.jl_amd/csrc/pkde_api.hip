@@ -540,6 +540,38 @@ int pkde_device_free_memory(pkde_ctx* ctx, uint64_t* free_bytes, uint64_t* total
   return PKDE_OK;
 }
 
+int pkde_find_grid(pkde_ctx* ctx, int ndims, const double* d_samples, int64_t n,
+                   const double* padding, double* lo_out, double* hi_out) {
+  if (!ctx || !d_samples || !lo_out || !hi_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (ndims < 1 || ndims > PKDE_MAX_DIMS || n < 1)
+    return fail(PKDE_INVALID_ARGUMENT, "ndims must be in 1..3 and n >= 1");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  double* dp = nullptr;
+  const size_t part = (size_t)kMomentBlocks * 2 * ndims;
+  int rc = dalloc((void**)&dp, (part + 2 * (size_t)ndims) * sizeof(double), "extrema");
+  double ext[2 * PKDE_MAX_DIMS];
+  if (!rc) {
+    launch_extrema(ndims, d_samples, n, dp, dp + part, ctx->stream);
+    rc = check_hip(hipGetLastError(), "extrema launch");
+  }
+  if (!rc) rc = check_hip(hipMemcpyAsync(ext, dp + part, 2 * ndims * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx->stream), "extrema");
+  if (!rc) rc = check_hip(hipStreamSynchronize(ctx->stream), "extrema");
+  (void)hipFree(dp);
+  if (rc) return rc;
+  for (int i = 0; i < ndims; ++i) {
+    const double lo = ext[2 * i], hi = ext[2 * i + 1];
+    if (!isfinite(lo) || !isfinite(hi))
+      return fail(PKDE_INVALID_ARGUMENT, "samples must be finite to find a grid");
+    // Grids.jl:275-281: padding 0.1 * |hi - lo| unless given; bounds +/- padding
+    const double pad = padding ? padding[i] : 0.1 * fabs(hi - lo);
+    lo_out[i] = lo - pad;
+    hi_out[i] = hi + pad;
+  }
+  return PKDE_OK;
+}
+
 int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
                               const double* hi, const double* h, const pkde_gradepro_params* params,
                               pkde_plan** out) {

@@ -321,6 +321,97 @@ struct AxisArgs {
   const cplx<T>* twg;
 };
 
+__host__ __device__ constexpr int gen_pencil_stride(int L) { return L + L / 8 + 1; }
+
+// Whether every pass of the plan f fits the wave-local mixed-radix FFT with
+// 64 lanes per pencil.
+bool wave_gen_fits(const FftFactors& f) {
+  if (f.L < 2 || f.L > 512) return false;
+  int Ns = 1;
+  for (int p = 0; p < f.np; ++p) {
+    const int R = f.rad[p];
+    const int per_lane = (f.L / R + 63) / 64;
+    if (wave_gen_nb(R) == 0 || per_lane > wave_gen_nb(R)) return false;
+    Ns *= R;
+  }
+  return Ns == f.L;
+}
+
+// Per-rung axis pass for lengths that are not powers of two and fit the
+// wave-local mixed-radix FFT (a.fac: radices 2, 3, 4, 5, 8): the block's
+// TK x TI pencils sit in LDS in the padded layout, each wavefront transforms
+// one pencil at a time, twiddles come from LDS.  Same indexing as k_axis_fft.
+template <typename T, int DIR>
+__global__ __launch_bounds__(kThreads) void k_axis_fft_wg(AxisArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* buf = (cplx<T>*)smem;
+  const int L = a.L;
+  const int PL = gen_pencil_stride(L);
+  cplx<T>* twl = buf + (int64_t)a.TK * a.TI * PL;
+  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
+  int64_t bid = blockIdx.x;
+  const int it = (int)(bid % a.itiles);
+  bid /= a.itiles;
+  const int kt = (int)(bid % a.ktiles);
+  bid /= a.ktiles;
+  const int64_t o = bid % a.O;
+  bid /= a.O;
+  const int fl = (int)(bid % a.nf);
+  const int rr = (int)(bid / a.nf);
+  const int f = a.fid0 + fl;
+  const int k0 = kt * a.TK;
+  const int TKe = (a.H1 - k0) < a.TK ? (a.H1 - k0) : a.TK;
+  const int64_t i0 = (int64_t)it * a.TI;
+  const int TI = a.TI;
+  const int npen = TKe * TI;
+  const T* gt = nullptr;
+  if (a.gM) gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  const cplx<T>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize;
+  cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
+  const int64_t obase = o * L * a.I;
+  const int tid = threadIdx.x;
+  const int total = npen * L;
+  for (int id = tid; id < total; id += kThreads) {
+    int kk, ii, e;
+    if (TI == 1) {
+      kk = id / L;
+      e = id - kk * L;
+      ii = 0;
+    } else {
+      ii = id % TI;
+      e = (id / TI) % L;
+      kk = id / (TI * L);
+    }
+    cplx<T> v = {(T)0, (T)0};
+    if (i0 + ii < a.I) {
+      const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+      v = src[(int64_t)(k0 + kk) * a.Q + q];
+      if (gt) v = cscale(v, gt[e]);
+    }
+    buf[(kk * TI + ii) * PL + pad8(e)] = v;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int p = wv; p < npen; p += kThreads / 64)
+    wave_fft_gen<DIR, T, 64>(buf + p * PL, lane, true, a.fac, twl);
+  __syncthreads();
+  for (int id = tid; id < total; id += kThreads) {
+    int kk, ii, e;
+    if (TI == 1) {
+      kk = id / L;
+      e = id - kk * L;
+      ii = 0;
+    } else {
+      ii = id % TI;
+      e = (id / TI) % L;
+      kk = id / (TI * L);
+    }
+    if (i0 + ii >= a.I) continue;
+    const int64_t q = obase + (int64_t)e * a.I + i0 + ii;
+    dst[(int64_t)(k0 + kk) * a.Q + q] = buf[(kk * TI + ii) * PL + pad8(e)];
+  }
+}
+
 // Generic form for pencil lengths outside the wave-local range or not powers
 // of two (GEN: mixed-radix FFT with the plan's twiddle table).  Inner tiles
 // may run past I (non-power-of-two axes); those pencils are not stored.
@@ -838,6 +929,19 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
     axis_wave_launch<TI, DIR>(a.log2L, (unsigned)blocks, lds, s, a);
     return;
   }
+  if (gen && twg && wave_gen_fits(g.facw[axis])) {
+    // wave-local mixed-radix form: 8 pencils per block (8 columns, or 8
+    // consecutive inner indices along the contiguous direction)
+    a.fac = g.facw[axis];
+    a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, 8);
+    a.TK = std::max(1, std::min(g.H1, 8 / a.TI));
+    a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+    a.itiles = (int)((a.I + a.TI - 1) / a.TI);
+    const int64_t blocks = (int64_t)R * nf * a.O * a.ktiles * a.itiles;
+    const size_t lds = ((size_t)a.TK * a.TI * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<TI>);
+    k_axis_fft_wg<TI, DIR><<<(unsigned)blocks, kThreads, lds, s>>>(a);
+    return;
+  }
   if (a.I == 1) {
     a.TI = 1;
   } else {
@@ -1353,22 +1457,6 @@ template void launch_ladder_rows<double>(const GridDev&, const cplx<double>*, in
 // wavefront, with the plan's twiddle tables staged in LDS.
 constexpr int kGenTK = 8;  // tile width of the tiled row layout (FP64)
 
-__host__ __device__ constexpr int gen_pencil_stride(int L) { return L + L / 8 + 1; }
-
-// Whether every pass of the plan f fits the wave-local mixed-radix FFT with
-// 64 lanes per pencil.
-bool wave_gen_fits(const FftFactors& f) {
-  if (f.L < 2 || f.L > 512) return false;
-  int Ns = 1;
-  for (int p = 0; p < f.np; ++p) {
-    const int R = f.rad[p];
-    const int per_lane = (f.L / R + 63) / 64;
-    if (wave_gen_nb(R) == 0 || per_lane > wave_gen_nb(R)) return false;
-    Ns *= R;
-  }
-  return Ns == f.L;
-}
-
 // One block per (rung, field, 8-column tile): load the tile's spectrum
 // columns along grid axis 2 with the rung's Gaussian factor, transform, store
 // the tile rows.  Columns past H1 up to the tile width are zero.
@@ -1867,6 +1955,59 @@ void launch_silverman(int N, const double* x, int64_t n, double prefactor, doubl
   k_moment_final<<<1, kThreads, 0, s>>>(N, kMomentBlocks, n, prefactor, partial, mean_out, nullptr);
   k_moment_partial<<<kMomentBlocks, kThreads, 0, s>>>(N, x, n, mean_out, partial);
   k_moment_final<<<1, kThreads, 0, s>>>(N, kMomentBlocks, n, prefactor, partial, nullptr, bw_out);
+}
+
+// Per-axis extrema of the samples (find_grid, Grids.jl:271-273: extrema of
+// each row of the ndims x n data); a NaN coordinate makes both extrema NaN,
+// as Julia's extrema does.  partial: [block][axis][min, max].
+__device__ __forceinline__ double nan_min(double a, double b) { return (b < a || b != b) ? b : a; }
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__global__ __launch_bounds__(kThreads) void k_extrema_partial(int N, const double* __restrict__ x,
+                                                              int64_t n, double* __restrict__ partial) {
+  __shared__ double lo_s[kThreads], hi_s[kThreads];
+  for (int i = 0; i < N; ++i) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+      const double v = x[k * N + i];
+      lo = nan_min(lo, v);
+      hi = nan_max(hi, v);
+    }
+    lo_s[threadIdx.x] = lo;
+    hi_s[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        lo_s[threadIdx.x] = nan_min(lo_s[threadIdx.x], lo_s[threadIdx.x + w]);
+        hi_s[threadIdx.x] = nan_max(hi_s[threadIdx.x], hi_s[threadIdx.x + w]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partial[(blockIdx.x * N + i) * 2] = lo_s[0];
+      partial[(blockIdx.x * N + i) * 2 + 1] = hi_s[0];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_extrema_final(int N, int nblocks, const double* __restrict__ partial,
+                                double* __restrict__ out) {
+  const int i = threadIdx.x;
+  if (i >= N) return;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int b = 0; b < nblocks; ++b) {
+    lo = nan_min(lo, partial[(b * N + i) * 2]);
+    hi = nan_max(hi, partial[(b * N + i) * 2 + 1]);
+  }
+  out[2 * i] = lo;
+  out[2 * i + 1] = hi;
+}
+
+void launch_extrema(int N, const double* x, int64_t n, double* partial, double* out, hipStream_t s) {
+  k_extrema_partial<<<kMomentBlocks, kThreads, 0, s>>>(N, x, n, partial);
+  k_extrema_final<<<1, 64, 0, s>>>(N, kMomentBlocks, partial, out);
 }
 
 // Rule-of-thumb propagation time of the :rot estimator (RoTEstimator.jl:

@@ -143,6 +143,9 @@ void launch_means_select(const GridDev& g, const cplx<T>* src, int64_t src_rung_
 void launch_silverman(int N, const double* x, int64_t n, double prefactor, double* partial,
                       double* mean_out, double* bw_out, hipStream_t s);
 constexpr int kMomentBlocks = 256;
+// Per-axis [min, max] of ndims x n samples into out[2 N] (NaN if any
+// coordinate of the axis is NaN); partial holds kMomentBlocks * 2 N doubles.
+void launch_extrema(int N, const double* x, int64_t n, double* partial, double* out, hipStream_t s);
 
 // Per-rung scalars (times, dlogt, VMR scaling, variance propagation factor,
 // steps_low / steps_over / eps) and the per-axis Gaussian tables.

@@ -173,16 +173,50 @@ def initialize_grid(ranges, device="rocm", **_):
 FIND_GRID_DEFAULT_POINTS = 150
 
 
+def _device_bounds(data, grid_padding):
+    """Padded bounds of samples already in device memory (a float64 tensor of
+    shape (n, ndims) on the GPU): pkde_find_grid reduces the extrema there.
+    torch carries its own HIP runtime: initialise torch's GPU state before
+    this package creates its context, so that both use one runtime."""
+    x = data if data.dim() == 2 else data.reshape(-1, 1)
+    if x.dtype != _torch_float64() or not x.is_contiguous():
+        raise ArgumentError("device samples must be a contiguous float64 tensor of shape (n, ndims)")
+    n, n_dims = int(x.shape[0]), int(x.shape[1])
+    lo = (_lib.C.c_double * n_dims)()
+    hi = (_lib.C.c_double * n_dims)()
+    pad = None
+    if grid_padding is not None:
+        pad = (_lib.C.c_double * n_dims)(*[float(p) for p in grid_padding])
+    _raise(_lib.load().pkde_find_grid(context().handle, n_dims, _lib.C.c_void_p(x.data_ptr()), n,
+                                       pad, lo, hi), "find_grid")
+    return [(lo[i], hi[i]) for i in range(n_dims)]
+
+
+def _torch_float64():
+    import torch
+    return torch.float64
+
+
+def _on_device(data) -> bool:
+    return bool(getattr(data, "is_cuda", False))
+
+
 def find_grid(data, grid_bounds=None, grid_dims=None, grid_steps=None, grid_padding=None,
               device="rocm"):
-    """Grids.jl:252-300: data extrema padded by 10% of the range per axis."""
-    x = _as_samples(data)
-    n_dims = x.shape[1]
-    if grid_bounds is None:
-        grid_bounds = [(float(x[:, i].min()), float(x[:, i].max())) for i in range(n_dims)]
-    if grid_padding is None:
-        grid_padding = [0.1 * abs(b[1] - b[0]) for b in grid_bounds]
-    gb = [(b[0] - p, b[1] + p) for b, p in zip(grid_bounds, grid_padding)]
+    """Grids.jl:252-300: data extrema padded by 10% of the range per axis.
+    Samples already in device memory (a torch tensor on the GPU) are reduced
+    on the device (pkde_find_grid); host samples on the host."""
+    if grid_bounds is None and _on_device(data):
+        gb = _device_bounds(data, grid_padding)
+        n_dims = len(gb)
+    else:
+        x = _as_samples(data)
+        n_dims = x.shape[1]
+        if grid_bounds is None:
+            grid_bounds = [(float(x[:, i].min()), float(x[:, i].max())) for i in range(n_dims)]
+        if grid_padding is None:
+            grid_padding = [0.1 * abs(b[1] - b[0]) for b in grid_bounds]
+        gb = [(b[0] - p, b[1] + p) for b, p in zip(grid_bounds, grid_padding)]
     if grid_steps is None and grid_dims is None:
         grid_dims = (FIND_GRID_DEFAULT_POINTS,) * n_dims
     if grid_dims is not None:

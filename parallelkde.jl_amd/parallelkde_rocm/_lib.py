@@ -26,7 +26,7 @@ PENDING = 7
 
 EXPORTED_SYMBOLS = [
     "pkde_version", "pkde_last_error", "pkde_gradepro_params_default",
-    "pkde_context_create", "pkde_context_destroy", "pkde_device_free_memory",
+    "pkde_context_create", "pkde_context_destroy", "pkde_device_free_memory", "pkde_find_grid",
     "pkde_gradepro_plan_create", "pkde_gradepro_plan_destroy", "pkde_gradepro_plan_footprint",
     "pkde_rot_plan_create",
     "pkde_gradepro_estimate", "pkde_gradepro_estimate_device", "pkde_gradepro_check",
@@ -112,6 +112,7 @@ def load() -> C.CDLL:
     L.pkde_context_create.argtypes = [C.c_int, C.POINTER(vp)]
     L.pkde_context_destroy.argtypes = [vp]
     L.pkde_device_free_memory.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.pkde_find_grid.argtypes = [vp, C.c_int, vp, C.c_int64, vp, vp, vp]
     L.pkde_gradepro_plan_create.argtypes = [vp, C.c_int, i64p, dp, dp, dp, C.POINTER(Params), C.POINTER(vp)]
     L.pkde_gradepro_plan_destroy.argtypes = [vp]
     L.pkde_rot_plan_create.argtypes = [vp, C.c_int, i64p, dp, dp, dp, C.c_int, C.c_int, C.POINTER(vp)]

@@ -315,6 +315,43 @@ def test_grid_with_large_prime_factor_rejected(pk):
         pk.estimate_density(est, "gradepro")
 
 
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_find_grid_on_device_matches_host(pk, L, n_dims):
+    """find_grid (Grids.jl:252-300) on samples already in device memory
+    (pkde_find_grid): the device extrema reduction gives the bounds of the
+    host computation exactly, with the default padding (10 % of the range)
+    and with a given one; non-finite samples are an ArgumentError."""
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library itself uses
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    rng = np.random.default_rng(20 + n_dims)
+    x = rng.standard_normal((30001, n_dims)) * np.array([1.0, 3.0, 0.5][:n_dims]) - 2.0
+    x = np.ascontiguousarray(x)
+    d = C.c_void_p()
+    assert hip.hipMalloc(C.byref(d), x.nbytes) == 0
+    try:
+        def device_bounds(host, padding):
+            assert hip.hipMemcpy(d, host.ctypes.data, host.nbytes, 1) == 0  # host to device
+            lo, hi = (C.c_double * n_dims)(), (C.c_double * n_dims)()
+            pad = None if padding is None else (C.c_double * n_dims)(*padding)
+            rc = L.pkde_find_grid(pk.context().handle, n_dims, d, host.shape[0], pad, lo, hi)
+            return rc, list(lo), list(hi)
+        for padding in (None, [0.25, 0.5, 1.0][:n_dims]):
+            rc, lo, hi = device_bounds(x, padding)
+            assert rc == 0
+            gh = pk.find_grid(x, grid_padding=padding)
+            gd = pk.initialize_grid([(lo[i], hi[i], 150) for i in range(n_dims)])
+            assert gd.n == gh.n == (150,) * n_dims
+            assert gd.lo == gh.lo and gd.hi == gh.hi and gd.h == gh.h
+        bad = x.copy()
+        bad[123, 0] = np.nan
+        rc, _, _ = device_bounds(bad, None)
+        assert rc == 1  # PKDE_INVALID_ARGUMENT
+    finally:
+        hip.hipFree(d)
+
+
 def test_deterministic_repeat(pk):
     """Fixed-point binning + fixed reduction orders: bitwise repeatable."""
     x = W.mixture_2d(30000, 9)
