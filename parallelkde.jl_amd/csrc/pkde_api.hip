@@ -324,9 +324,12 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       // so the axis-3 pass reads whole pencils and writes [q][k1] rows.
       if constexpr (sizeof(T) == 8) {
         if (rows_gen) {
-          if (!launch_axis_gen_rows(g, spec, Y, per_rung, 2 * B, R, r0, gM, gV, B, twg, s) ||
-              !launch_ladder_rows_gen(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, twg,
-                                      d.invk, d.vmr, s))
+          const bool passes =
+              N == 2 ? launch_axis_gen_rows(g, spec, Y, per_rung, 2 * B, R, r0, gM, gV, B, twg, s)
+                     : launch_gen3_passes(g, spec, Y2, Y, per_rung, 2 * B, R, r0, gM, gV, B, twg,
+                                          s);
+          if (!passes || !launch_ladder_rows_gen(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV,
+                                                 d.vscale, twg, d.invk, d.vmr, s))
             return fail(PKDE_DEVICE_ERROR, "mixed-radix ladder layout mismatch");
           continue;
         }

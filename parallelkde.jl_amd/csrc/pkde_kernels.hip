@@ -1032,6 +1032,7 @@ struct VmrArgs {
   // tiled row layout (rows kernel): tile width 2^log2TKy, tile stride tstride;
   // rows3: rows are q' = j3 + n3 * j2 (3-D), grid row q = j2 + n2 * j3
   int log2TKy, rows3, log2n2, log2n3;
+  int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -1504,8 +1505,142 @@ __global__ __launch_bounds__(kThreads) void k_axis_gen_rows(AxisArgs<double> a) 
 }
 
 bool ladder_gen_rows_ok(const GridDev& g) {
-  return g.N == 2 && !g.pow2 && g.TKy == kGenTK && (g.n[0] + 63) / 64 <= 8 &&
-         wave_gen_fits(g.facw[0]) && wave_gen_fits(g.facw[1]);
+  return (g.N == 2 || g.N == 3) && !g.pow2 && g.TKy == kGenTK && (g.n[0] + 63) / 64 <= 8 &&
+         wave_gen_fits(g.facw[0]) && wave_gen_fits(g.facw[1]) &&
+         (g.N == 2 || wave_gen_fits(g.facw[2]));
+}
+
+// 3-D, first pass (grid axis 2, length n2 = a.I): one block per (rung, field,
+// k1, 8 consecutive j3); pencils along j2 are contiguous in the spectra
+// [k1][j2 + n2 j3]; the result is stored as [k1][j2][j3], 8 consecutive j3
+// (one 128-byte run) per (k1, j2).
+__global__ __launch_bounds__(kThreads) void k_gen3_pass1(AxisArgs<double> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* buf = (cplx<double>*)smem;
+  const int L = a.L;  // n2
+  const int PL = gen_pencil_stride(L);
+  cplx<double>* twl = buf + kGenTK * PL;
+  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
+  const int64_t n3 = a.O;
+  const int jtiles = (int)((n3 + kGenTK - 1) / kGenTK);
+  uint32_t bid = blockIdx.x;
+  const int jt = (int)(bid % (uint32_t)jtiles);
+  bid /= (uint32_t)jtiles;
+  const int k1 = (int)(bid % (uint32_t)a.H1);
+  bid /= (uint32_t)a.H1;
+  const int f = a.fid0 + (int)(bid % (uint32_t)a.nf);
+  const int rr = (int)(bid / (uint32_t)a.nf);
+  const int j30 = jt * kGenTK;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  const cplx<double>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize + (int64_t)k1 * a.Q;
+  for (int id = tid; id < kGenTK * L; id += kThreads) {
+    const int j = id / L, e = id - j * L;  // pencil j: j3 = j30 + j
+    cplx<double> v = {0.0, 0.0};
+    if (j30 + j < n3) v = cscale(src[(int64_t)(j30 + j) * L + e], gt[e]);
+    buf[j * PL + pad8(e)] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int j = 2 * wv + c;
+    if (j30 + j < n3) wave_fft_gen<1, double, 64>(buf + j * PL, lane, true, a.fac, twl);
+  }
+  __syncthreads();
+  cplx<double>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
+                      (int64_t)k1 * a.Q;
+  for (int id = tid; id < kGenTK * L; id += kThreads) {
+    const int j = id & (kGenTK - 1), e = id >> 3;
+    if (j30 + j < n3) dst[(int64_t)e * n3 + j30 + j] = buf[j * PL + pad8(e)];
+  }
+}
+
+// 3-D, second pass (grid axis 3, length n3): one block per (rung, field, j2,
+// 8-column k1 tile); pencils along j3 are contiguous in [k1][j2][j3]; the
+// result goes to the tiled row layout with rows q' = j3 + n3 j2 (whole
+// 128-byte lines: columns past H1 are zero).
+__global__ __launch_bounds__(kThreads) void k_gen3_pass2(AxisArgs<double> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* buf = (cplx<double>*)smem;
+  const int L = a.L;  // n3
+  const int PL = gen_pencil_stride(L);
+  cplx<double>* twl = buf + kGenTK * PL;
+  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
+  uint32_t bid = blockIdx.x;
+  const int kt = (int)(bid % (uint32_t)a.ktiles);
+  bid /= (uint32_t)a.ktiles;
+  const int64_t j2 = bid % (uint32_t)a.I;
+  bid /= (uint32_t)a.I;
+  const int f = a.fid0 + (int)(bid % (uint32_t)a.nf);
+  const int rr = (int)(bid / (uint32_t)a.nf);
+  const int k0 = kt * kGenTK;
+  const int TKe = (a.H1 - k0) < kGenTK ? (a.H1 - k0) : kGenTK;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
+  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)(f - a.src_fid0) * a.fsize +
+                            j2 * L;
+  for (int id = tid; id < kGenTK * L; id += kThreads) {
+    const int kk = id / L, e = id - kk * L;
+    cplx<double> v = {0.0, 0.0};
+    if (kk < TKe) v = cscale(src[(int64_t)(k0 + kk) * a.Q + e], gt[e]);
+    buf[kk * PL + pad8(e)] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int kk = 2 * wv + c;
+    if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
+  }
+  __syncthreads();
+  cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
+                       (int64_t)kt * a.Q * kGenTK + j2 * L * kGenTK;
+  for (int id = tid; id < kGenTK * L; id += kThreads)
+    tile[id] = buf[(id & (kGenTK - 1)) * PL + pad8(id >> 3)];
+}
+
+bool launch_gen3_passes(const GridDev& g, const cplx<double>* spec, cplx<double>* y2,
+                        cplx<double>* y, int64_t rung_stride, int nf, int R, int r0,
+                        const double* gM, const double* gV, int fsplit, const double* twg,
+                        hipStream_t s) {
+  if (g.N != 3 || !ladder_gen_rows_ok(g)) return false;
+  AxisArgs<double> a{};
+  a.src = spec;
+  a.dst = y2;
+  a.dst_rs = rung_stride;
+  a.fsize = (int64_t)g.H1 * g.Q;
+  a.dst_fsize = a.fsize;
+  a.nf = nf;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.fsplit = fsplit;
+  a.ksum = g.ksum;
+  a.Q = g.Q;
+  a.H1 = g.H1;
+  a.I = g.n[1];
+  a.O = g.n[2];
+  // pass 1: axis 2
+  a.L = g.n[1];
+  a.goff = g.koff[1];
+  a.fac = g.facw[1];
+  a.twg = (const cplx<double>*)twg + g.koff[1];
+  const int jtiles = (g.n[2] + kGenTK - 1) / kGenTK;
+  size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
+  k_gen3_pass1<<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
+  // pass 2: axis 3, into the tiled rows
+  a.src = y2;
+  a.src_rs = rung_stride;
+  a.dst = y;
+  a.dst_fsize = g.fsY;
+  a.L = g.n[2];
+  a.goff = g.koff[2];
+  a.fac = g.facw[2];
+  a.twg = (const cplx<double>*)twg + g.koff[2];
+  a.ktiles = g.ktY;
+  lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
+  k_gen3_pass2<<<(unsigned)((int64_t)R * nf * g.n[1] * g.ktY), kThreads, lds, s>>>(a);
+  return true;
 }
 
 bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<double>* dst,
@@ -1646,12 +1781,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows_gen(VmrArgs<double> a)
     wave_sync();
   }
   const double sc = a.vscale[r];
+  // 3-D: layout rows q' = j3 + n3 j2 -> grid row j2 + n2 j3
+  const int64_t qg = a.rows3 ? q / a.n3r + (q % a.n3r) * a.n2r : q;
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int j1 = lane + 64 * u;
     if (j1 < n1) {
       const double vv = sc * m2[u] / (double)(B - 1);
-      a.vmr[(int64_t)r * a.G + q * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+      a.vmr[(int64_t)r * a.G + qg * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
     }
   }
 }
@@ -1686,6 +1823,9 @@ bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t s
   a.H1 = g.H1;
   a.fac = g.facw[0];
   a.twg = (const cplx<double>*)twg + g.koff[0];
+  a.rows3 = g.N == 3;
+  a.n2r = g.n[1];
+  a.n3r = g.N == 3 ? g.n[2] : 1;
   const int64_t waves = (int64_t)R * g.Q;
   const int wpb = kThreads / 64;
   const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);

@@ -78,12 +78,18 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
 // 2-D ladder for axis lengths that are not powers of two (FP64): a per-rung
 // axis-2 pass into the tiled row layout (8-column tiles) and the row kernel on
 // the wave-local mixed-radix FFT.  ladder_gen_rows_ok: the grid qualifies
-// (N == 2, some axis not a power of two, both axes <= 512 with radices the
-// wave-local FFT covers, GridDev::TKy == 8).
+// (N == 2 or 3, some axis not a power of two, every axis <= 512 with radices
+// the wave-local FFT covers, GridDev::TKy == 8).
 bool ladder_gen_rows_ok(const GridDev& g);
 bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<double>* dst,
                           int64_t dst_rung_stride, int nf, int R, int r0, const double* gM,
                           const double* gV, int fsplit, const double* twg, hipStream_t s);
+// 3-D: both inverse axis passes of a rung chunk (axis 2 into y2 as
+// [k1][j2][j3], axis 3 from there into the tiled rows of y).
+bool launch_gen3_passes(const GridDev& g, const cplx<double>* spec, cplx<double>* y2,
+                        cplx<double>* y, int64_t rung_stride, int nf, int R, int r0,
+                        const double* gM, const double* gV, int fsplit, const double* twg,
+                        hipStream_t s);
 bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride,
                             int B, int64_t n_samples, int R, int r0, const double* gM,
                             const double* gV, const double* cV, const double* vscale,

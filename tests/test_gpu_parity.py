@@ -221,6 +221,10 @@ CASES = [
     ("2d-150x96", ([-6.0, -6.0], [6.0, 6.0], [150, 96]), lambda: W.mixture_2d(20000, 7), 30, 100),
     ("2d-64x48", ([-6.0, -6.0], [6.0, 6.0], [64, 48]), lambda: W.mixture_2d(20000, 8), 30, 100),
     ("3d-21x20x18", ([-4.0] * 3, [4.0] * 3, [21, 20, 18]), lambda: W.correlated_3d(20000, 9), 20, 60),
+    # radices 2, 3, 4, 5, 8 only: the wave-local mixed-radix ladder; an odd
+    # row length has no Nyquist column
+    ("3d-24x20x18", ([-4.0] * 3, [4.0] * 3, [24, 20, 18]), lambda: W.correlated_3d(20000, 11), 20, 60),
+    ("2d-45x40", ([-6.0, -6.0], [6.0, 6.0], [45, 40]), lambda: W.mixture_2d(20000, 12), 30, 100),
 ]
 
 
