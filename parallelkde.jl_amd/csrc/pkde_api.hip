@@ -124,6 +124,7 @@ uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   b += (uint64_t)F * fs * 16;                     // FP64 spectra
   if (!pl->prec) b += (uint64_t)F * fs * 8;       // FP32 spectra
   b += (uint64_t)pl->S * G * 8;                   // log-VMR table
+  if (G <= kConvTermsMaxG) b += (uint64_t)pl->S * G * 16;  // convergence terms
   b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
   b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
   if (pl->g.N >= 2) b += (uint64_t)y_units(pl) * y_part_bytes(pl);
@@ -164,6 +165,11 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
     d.gM = nullptr; d.gV = nullptr;
     const int S = pl->S;
     if ((rc = dalloc((void**)&d.vmr, (size_t)S * G * 8, "vmr table"))) return rc;
+    dfree(d.conv);
+    d.conv = nullptr;
+    if (G <= kConvTermsMaxG &&
+        (rc = dalloc((void**)&d.conv, (size_t)S * G * 16, "convergence terms")))
+      return rc;
     if ((rc = dalloc((void**)&d.times, (size_t)S * PKDE_MAX_DIMS * 8, "times"))) return rc;
     if ((rc = dalloc((void**)&d.dlogt, (size_t)S * 8, "dlogt"))) return rc;
     if ((rc = dalloc((void**)&d.vscale, (size_t)S * 8, "vscale"))) return rc;
@@ -203,7 +209,7 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
 
 static void plan_free(pkde_plan* pl) {
   PlanBuffers& d = pl->d;
-  void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.chosen, d.sel, d.density, d.err,
+  void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.conv, d.chosen, d.sel, d.density, d.err,
                 d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
                 d.gM, d.gV, d.invk, d.binwork, d.twg64, d.twg32};
   for (void* p : ps) dfree(p);
@@ -361,7 +367,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
 
   // ---- convergence + means at the chosen rung ----
   int16_t* chosen = d_chosen ? d_chosen : d.chosen;
-  launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, s);
+  launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, d.conv, s);
   const int rc = enqueue_means<T>(pl, spec, gM, tw, d_density, s);
   if (rc) return rc;
   if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
@@ -884,9 +890,11 @@ int pkde_gradepro_convergence(pkde_ctx* ctx, int64_t G, int32_t S, const double*
     return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-  double *dv = nullptr, *dd = nullptr, *dp = nullptr;
+  double *dv = nullptr, *dd = nullptr, *dp = nullptr, *dt = nullptr;
   int16_t *dc = nullptr, *ds = nullptr;
   int rc = dalloc((void**)&dv, (size_t)S * G * 8, "vmr");
+  // the form an estimate of this grid size uses
+  if (!rc && G <= kConvTermsMaxG) rc = dalloc((void**)&dt, (size_t)S * G * 16, "terms");
   if (!rc) rc = dalloc((void**)&dd, (size_t)S * 8, "dlogt");
   if (!rc) rc = dalloc((void**)&dp, 256, "params");
   if (!rc) rc = dalloc((void**)&dc, (size_t)G * 2, "chosen");
@@ -896,11 +904,11 @@ int pkde_gradepro_convergence(pkde_ctx* ctx, int64_t G, int32_t S, const double*
   if (!rc) rc = check_hip(hipMemcpy(dd, dlogt, (size_t)S * 8, hipMemcpyHostToDevice), "upload");
   if (!rc) rc = check_hip(hipMemcpy(dp, par, sizeof(par), hipMemcpyHostToDevice), "upload");
   if (!rc) {
-    launch_convergence(G, S, dv, dd, dp, dc, ds, ctx->stream);
+    launch_convergence(G, S, dv, dd, dp, dc, ds, dt, ctx->stream);
     rc = check_hip(hipStreamSynchronize(ctx->stream), "convergence");
   }
   if (!rc) rc = check_hip(hipMemcpy(chosen_out, dc, (size_t)G * 2, hipMemcpyDeviceToHost), "download");
-  dfree(dv); dfree(dd); dfree(dp); dfree(dc); dfree(ds);
+  dfree(dv); dfree(dd); dfree(dp); dfree(dc); dfree(ds); dfree(dt);
   return rc;
 }
 

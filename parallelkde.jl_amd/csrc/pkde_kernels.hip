@@ -1848,6 +1848,55 @@ bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t s
 // history rotation of update_density_state_vmr! (GradeProEstimator.jl:617-625);
 // state stays in registers.  chosen = rung of the last density assignment, or
 // S if the point never converged (replace_nans!, GradeProEstimator.jl:1053-1061).
+// The two terms of find_stability (DirectSpace.jl:1092-1100, first and
+// second differences of the log-VMR over dlogt) for the value f at one rung
+// and its two predecessors (NaN before the first rungs).
+__device__ __forceinline__ void stability_terms(double f, double f1, double f2, double dl,
+                                                double& d1, double& d2) {
+  d1 = log(fabs(__ddiv_rn(__dsub_rn(f, f1), __dmul_rn(2.0, dl))));
+  d2 = log(fabs(__ddiv_rn(__dadd_rn(__dsub_rn(f, __dmul_rn(2.0, f1)), f2), __dmul_rn(dl, dl))));
+}
+
+// One rung of the per-point state machine (find_stability and the state
+// update, DirectSpace.jl:975-1031, GradeProEstimator.jl:617-636).
+struct StabilityState {
+  double cur_min = NAN;
+  unsigned cl = 0, co = 0;
+  bool low = false;
+  int ch;
+};
+
+__device__ __forceinline__ void stability_step(StabilityState& st, double d1, double d2, double eps,
+                                               int steps_low, int steps_over, int r) {
+  bool more_stable = false;
+  double new_min = NAN;
+  if (!st.low) {
+    if (d2 < eps) {
+      if (((d1 < st.cur_min) || isnan(st.cur_min)) && ((int)st.co <= steps_over)) {
+        more_stable = true;
+        new_min = d1;
+        st.co = 0;
+      } else {
+        st.co = (st.co + 1) & 0xFFFFu;
+      }
+    } else {
+      st.cl = (st.cl + 1) & 0xFFFFu;
+      if ((int)st.cl > steps_low) {
+        st.low = true;
+        st.cur_min = NAN;
+      }
+    }
+  } else {
+    if ((d2 < eps) && isnan(st.cur_min)) {
+      more_stable = true;
+      new_min = d1;
+    }
+  }
+  if (isnan(new_min)) new_min = st.cur_min;
+  st.cur_min = new_min;
+  if (more_stable) st.ch = r;
+}
+
 __global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, const double* __restrict__ vmr,
                                                           const double* __restrict__ dlogt,
                                                           const double* __restrict__ params,
@@ -1858,42 +1907,13 @@ __global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, cons
   const int steps_over = (int)params[2];
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < G;
        p += (int64_t)gridDim.x * blockDim.x) {
-    double f1 = NAN, f2 = NAN, cur_min = NAN;
-    unsigned cl = 0, co = 0;
-    bool low = false;
-    int ch = S;
+    double f1 = NAN, f2 = NAN;
+    StabilityState st;
+    st.ch = S;
     auto step = [&](const double f, const int r) {
-      const double dl = dlogt[r];
-      const double d1 = log(fabs(__ddiv_rn(__dsub_rn(f, f1), __dmul_rn(2.0, dl))));
-      const double d2 = log(fabs(__ddiv_rn(__dadd_rn(__dsub_rn(f, __dmul_rn(2.0, f1)), f2),
-                                           __dmul_rn(dl, dl))));
-      bool more_stable = false;
-      double new_min = NAN;
-      if (!low) {
-        if (d2 < eps) {
-          if (((d1 < cur_min) || isnan(cur_min)) && ((int)co <= steps_over)) {
-            more_stable = true;
-            new_min = d1;
-            co = 0;
-          } else {
-            co = (co + 1) & 0xFFFFu;
-          }
-        } else {
-          cl = (cl + 1) & 0xFFFFu;
-          if ((int)cl > steps_low) {
-            low = true;
-            cur_min = NAN;
-          }
-        }
-      } else {
-        if ((d2 < eps) && isnan(cur_min)) {
-          more_stable = true;
-          new_min = d1;
-        }
-      }
-      if (isnan(new_min)) new_min = cur_min;
-      cur_min = new_min;
-      if (more_stable) ch = r;
+      double d1, d2;
+      stability_terms(f, f1, f2, dlogt[r], d1, d2);
+      stability_step(st, d1, d2, eps, steps_low, steps_over, r);
       f2 = f1;
       f1 = f;
     };
@@ -1918,13 +1938,76 @@ __global__ __launch_bounds__(kThreads) void k_convergence(int64_t G, int S, cons
       for (int i = 0; i < RB; ++i)
         if (r0 + i < S) step(cur[i], r0 + i);
     }
-    chosen[p] = (int16_t)ch;
-    sel[p] = (int16_t)(ch == S ? S - 1 : ch);
+    chosen[p] = (int16_t)st.ch;
+    sel[p] = (int16_t)(st.ch == S ? S - 1 : st.ch);
   }
 }
 
+// Small grids (a few waves in all): the logarithms and divisions of every
+// (rung, point) are computed in parallel first (k_conv_terms), so the serial
+// walk over the rungs (k_convergence_walk) only compares.  Same expressions,
+// same results as k_convergence.
+__global__ __launch_bounds__(kThreads) void k_conv_terms(int64_t G, int S, const double* __restrict__ vmr,
+                                                         const double* __restrict__ dlogt,
+                                                         double* __restrict__ terms) {
+  const int64_t total = (int64_t)S * G;
+  for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(id / G);
+    const double f = vmr[id];
+    const double f1 = r >= 1 ? vmr[id - G] : (double)NAN;
+    const double f2 = r >= 2 ? vmr[id - 2 * G] : (double)NAN;
+    double d1, d2;
+    stability_terms(f, f1, f2, dlogt[r], d1, d2);
+    terms[2 * id] = d1;
+    terms[2 * id + 1] = d2;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_convergence_walk(int64_t G, int S, const double* __restrict__ terms,
+                                                         const double* __restrict__ params,
+                                                         int16_t* __restrict__ chosen,
+                                                         int16_t* __restrict__ sel) {
+  const double eps = params[0];
+  const int steps_low = (int)params[1];
+  const int steps_over = (int)params[2];
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= G) return;
+  StabilityState st;
+  st.ch = S;
+  constexpr int RB = 16;
+  double2 nxt[RB];
+  const double2* t2 = (const double2*)terms;
+  auto fetch = [&](int r0) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = r0 + i < S ? r0 + i : S - 1;
+      nxt[i] = t2[(int64_t)r * G + p];
+    }
+  };
+  fetch(0);
+  for (int r0 = 0; r0 < S; r0 += RB) {
+    double2 cur[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) cur[i] = nxt[i];
+    if (r0 + RB < S) fetch(r0 + RB);
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if (r0 + i < S) stability_step(st, cur[i].x, cur[i].y, eps, steps_low, steps_over, r0 + i);
+  }
+  chosen[p] = (int16_t)st.ch;
+  sel[p] = (int16_t)(st.ch == S ? S - 1 : st.ch);
+}
+
 void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
-                        const double* params, int16_t* chosen, int16_t* sel, hipStream_t s) {
+                        const double* params, int16_t* chosen, int16_t* sel, double* terms,
+                        hipStream_t s) {
+  if (terms) {
+    k_conv_terms<<<grid_for((int64_t)S * G, kThreads, 1 << 16), kThreads, 0, s>>>(G, S, vmr, dlogt,
+                                                                                 terms);
+    k_convergence_walk<<<(unsigned)((G + 63) / 64), 64, 0, s>>>(G, S, terms, params, chosen, sel);
+    return;
+  }
   k_convergence<<<grid_for(G, kThreads, 1 << 16), kThreads, 0, s>>>(G, S, vmr, dlogt, params, chosen,
                                                                      sel);
 }

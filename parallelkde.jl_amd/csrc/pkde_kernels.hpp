@@ -135,9 +135,13 @@ void launch_gauss_tables(const GridDev& g, int S, const double* times, const dou
                          T* gV, hipStream_t s);
 
 // Convergence state machine over all rungs -> chosen rung, selection rung.
+// terms (2 S G doubles, or NULL): scratch for the small-grid form, which
+// computes the logarithmic terms of every (rung, point) in parallel first.
 void launch_convergence(int64_t G, int S, const double* vmr, const double* dlogt,
                         const double* params /* eps, steps_low, steps_over */,
-                        int16_t* chosen, int16_t* sel, hipStream_t s);
+                        int16_t* chosen, int16_t* sel, double* terms, hipStream_t s);
+// Grids up to this many points use the small-grid convergence form.
+constexpr int64_t kConvTermsMaxG = 16384;
 
 // Full-sample means at each point's selected rung -> density.
 template <typename T>

@@ -58,6 +58,7 @@ struct PlanBuffers {
   void* Y = nullptr;              // rung-chunk buffer
   int64_t Y_elems = 0;
   double* vmr = nullptr;  // S x G
+  double* conv = nullptr;  // S x G x 2 convergence terms (small grids only)
   int16_t* chosen = nullptr;
   int16_t* sel = nullptr;
   double* density = nullptr;

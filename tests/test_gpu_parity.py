@@ -159,8 +159,10 @@ def test_convergence_kernel_matches_oracle(L, pk):
     find_stability applied rung by rung (DirectSpace.jl:975-1031)."""
     say("[convergence] random tables")
     rng = np.random.default_rng(5)
+    # grids up to 16384 points take the small-grid form (terms computed in
+    # parallel first), larger ones the single pass
     for trial, (G, S, eps, sl, so) in enumerate([(5000, 60, 2.0, 0, 25), (4000, 80, -0.5, 15, 25),
-                                                  (3000, 40, -0.5, 3, 5)]):
+                                                  (3000, 40, -0.5, 3, 5), (20000, 50, -0.5, 15, 25)]):
         base = np.cumsum(rng.normal(0, 0.05, (S, G)), axis=0) + rng.normal(0, 1, G)
         vmr = base + rng.normal(0, 0.02, (S, G))
         vmr[rng.random((S, G)) < 0.01] = np.nan
