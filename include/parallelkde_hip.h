@@ -58,7 +58,10 @@ typedef struct pkde_server pkde_server;
 typedef struct {
   int32_t n_bootstraps;                /* 100 (:705) */
   int32_t n_steps;                     /* 250 (:902); <= 0 -> bw_step if finite */
-  double bw_step;                      /* NaN: unset (:887-899) */
+  double bw_step;                      /* NaN: unset (:887-899); with the
+                                          Silverman bw_final the ladder length
+                                          is set per estimate from the samples
+                                          (that estimate waits for them) */
   double bw_final[PKDE_MAX_DIMS];      /* NaN: Silverman rule (:771-775) */
   double alpha_s;                      /* NaN: 0.0 (1-D) / 0.06 (>=2-D) (:754-763) */
   double alpha_os;                     /* NaN: 0.1 (:764-766) */

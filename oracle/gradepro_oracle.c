@@ -277,19 +277,20 @@ typedef struct {
   double *w; /* w[2k], w[2k+1] = cos, sin of 2 pi k / n */
 } og_fft1;
 
-static int og_fft_reversed = 0;
-void og_set_fft_order(int reversed) { og_fft_reversed = reversed ? 1 : 0; }
+static int og_fft_mode = 0;
+void og_set_fft_order(int mode) { og_fft_mode = (mode == 1 || mode == 2) ? mode : 0; }
 
 static int og_fft1_init(og_fft1 *p, int64_t n) {
   p->n = n;
   p->nf = 0;
   int64_t m = n;
-  while (m % 4 == 0) { p->factors[p->nf++] = 4; m /= 4; }
+  if (og_fft_mode != 2)
+    while (m % 4 == 0) { p->factors[p->nf++] = 4; m /= 4; }
   while (m % 2 == 0) { p->factors[p->nf++] = 2; m /= 2; }
   for (int64_t f = 3; f * f <= m; f += 2)
     while (m % f == 0) { p->factors[p->nf++] = f; m /= f; }
   if (m > 1) p->factors[p->nf++] = m;
-  if (og_fft_reversed)
+  if (og_fft_mode == 1)
     for (int i = 0, j = p->nf - 1; i < j; ++i, --j) {
       const int64_t t = p->factors[i];
       p->factors[i] = p->factors[j];

@@ -94,11 +94,13 @@ int og_dirac_sequences(const og_grid *g, const double *samples, int64_t n,
 int og_fft_nd(int ndims, const int64_t *n, double *data, int sign);
 /* Selects the radix order of og_fft_nd for the calling process: 0 (default)
  * applies radix 4, then 2, then odd primes ascending; 1 applies the same
- * factors in reverse order.  Both are exact DFTs in exact arithmetic; the
- * second sums in a different order, so running a whole estimate under each
+ * factors in reverse order; 2 splits the power of two into radix-2 passes
+ * (then odd primes ascending).  All are exact DFTs in exact arithmetic; the
+ * others sum in a different order, so running a whole estimate under each
  * measures how far FP64 rounding alone moves the reference's result (its
- * rounding envelope).  Not thread-safe: set it between estimates. */
-void og_set_fft_order(int reversed);
+ * rounding envelope; mode 2 also perturbs lengths whose factors are all 4).
+ * Not thread-safe: set it between estimates. */
+void og_set_fft_order(int mode);
 /* propagate_time_cpu! (FourierSpace.jl:215-262) on one complex field. vars may
  * be NULL (means-only form). */
 void og_propagate(const og_grid *g, const double *kgrid_axes, const double *t,

@@ -85,12 +85,15 @@ static int ilog2(int64_t v) {
 
 static size_t real_size(int prec) { return prec ? sizeof(double) : sizeof(float); }
 
-// fixed-point fraction bits: deposits of n draws must fit in int64
+// Fixed-point fraction bits.  A draw deposits round((w / prod h) 2^fb) per
+// corner, with w / prod h in [0, 1] and the corners of one draw summing to 1,
+// so a cell of one field holds at most n 2^fb: fb = 61 - ceil(log2(n + 1))
+// keeps every cell below 2^61.  Each deposit is exact to 2^-(fb+1) of a whole
+// draw, so the most bits the bound allows is the closest match to FP64 sums.
 static int frac_bits_for(int64_t n) {
   int lg = 0;
   while ((1ll << lg) <= n) ++lg;
-  int fb = 61 - lg;
-  return fb > 40 ? 40 : fb;
+  return 61 - lg;
 }
 
 static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; }
@@ -431,12 +434,36 @@ static int enqueue_rot(pkde_plan* pl, const double* d_x, int64_t n, double* d_de
   return check_hip(hipGetLastError(), "kernel launch");
 }
 
+// bw_step with the Silverman bw_final (GradeProEstimator.jl:887-899): the
+// ladder length n_steps = min_i floor(bw_final_i / bw_step) needs the
+// samples' bandwidths, computed on the device and read back (this estimate
+// waits for them), then the plan's S-sized buffers are grown if needed.
+static int ladder_from_data(pkde_plan* pl, const double* d_x, int64_t n, hipStream_t s) {
+  PlanBuffers& d = pl->d;
+  const int N = pl->g.N;
+  const double pref = pow(4.0 / ((double)(N + 2) * (double)n), 1.0 / (double)(N + 4));
+  launch_silverman(N, d_x, n, pref, d.partial, d.mean, d.bw, s);
+  double bw[PKDE_MAX_DIMS];
+  PKDE_TRY(hipMemcpyAsync(bw, d.bw, N * sizeof(double), hipMemcpyDeviceToHost, s), "bandwidth");
+  PKDE_TRY(hipStreamSynchronize(s), "bandwidth");
+  double m = INFINITY;
+  for (int i = 0; i < N; ++i) {
+    if (!(pl->steps_or_step <= bw[i]))  // also catches a NaN bandwidth
+      return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
+    m = std::min(m, floor(bw[i] / pl->steps_or_step));
+  }
+  if (m + 1 > 32000) return fail(PKDE_INVALID_ARGUMENT, "too many bandwidth steps");
+  pl->S = (int)m + 1;
+  return plan_reserve(pl, n, false);
+}
+
 int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
                  double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
   if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
   if (n >= (1ll << 31)) return fail(PKDE_INVALID_ARGUMENT, "more than 2^31-1 samples");
   int rc = plan_reserve(pl, n, false);
   if (rc) return rc;
+  if (pl->S_from_data && pl->kind == 0 && (rc = ladder_from_data(pl, d_x, n, s))) return rc;
   pl->last_n = n;
   if (pl->kind == 1)
     return pl->prec ? enqueue_rot<double>(pl, d_x, n, d_density, s, timed)
@@ -675,11 +702,17 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     pl->steps_mode = 1;
     pl->steps_or_step = (double)p.n_steps;
     pl->S = p.n_steps + 1;
-  } else if (isfinite(p.bw_step)) {
-    if (!pl->bw_given) {
+  } else if (isfinite(p.bw_step) && !pl->bw_given) {
+    // the ladder length floor(bw_final / bw_step) waits for the samples
+    if (!(p.bw_step > 0.0)) {
       delete pl;
-      return fail(PKDE_UNSUPPORTED, "bw_step with the Silverman bw_final needs bw_final on :rocm");
+      return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
     }
+    pl->steps_mode = 0;
+    pl->steps_or_step = p.bw_step;
+    pl->S = 251;
+    pl->S_from_data = true;
+  } else if (isfinite(p.bw_step)) {
     double m = INFINITY;
     for (int i = 0; i < ndims; ++i) {
       if (p.bw_step > pl->bw_host[i] || p.bw_step <= 0.0) {

@@ -93,6 +93,9 @@ struct pkde_plan {
   int B = 100;
   int S = 251;            // rungs (n_steps + 1)
   int steps_mode = 1;     // 1: n_steps, 0: bw_step
+  // bw_step with the Silverman bw_final: S depends on the samples and is set
+  // by each estimate (the plan keeps S = 251 for footprints until then)
+  bool S_from_data = false;
   double steps_or_step = 250.0;
   bool bw_given = false;
   double bw_host[PKDE_MAX_DIMS] = {0, 0, 0};

@@ -168,23 +168,26 @@ def fft_nd(arr: np.ndarray, sign: int) -> np.ndarray:
     return buf.view(np.complex128).reshape(a.shape)
 
 
-def set_fft_order(reversed_: bool):
-    """Radix order of the oracle's FFT (og_set_fft_order)."""
-    lib().og_set_fft_order(1 if reversed_ else 0)
+def set_fft_order(mode: int):
+    """Radix order of the oracle's FFT (og_set_fft_order): 0 default,
+    1 reversed, 2 radix-2 passes for the power of two."""
+    lib().og_set_fft_order(int(mode))
 
 
 def rounding_envelope(g: OgGrid, samples_col, idx_bs, **kw):
     """The oracle's estimate under its default FFT radix order and under the
-    reversed order, and the distance between the two: how far FP64 rounding
-    alone moves the reference's result on this input.  Returns (default
-    result, reversed result)."""
+    two other summation orders: how far FP64 rounding alone moves the
+    reference's result on this input.  Returns (default result, [results
+    under the other orders])."""
     r0 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
-    set_fft_order(True)
+    others = []
     try:
-        r1 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
+        for mode in (1, 2):
+            set_fft_order(mode)
+            others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
     finally:
-        set_fft_order(False)
-    return r0, r1
+        set_fft_order(0)
+    return r0, others
 
 
 def dirac_sequences(g: OgGrid, samples_col: np.ndarray, idx_bs: np.ndarray, with_squares=True, threads=1):
