@@ -77,6 +77,17 @@ static void factorize_wave(int L, FftFactors& f) {
   if (n != 1) f.np = 0;
 }
 
+// The power-of-two row path of the 2-D/3-D ladder: every axis a power of
+// two, Q >= 64 rows, 16 <= n1 <= 512 and the other axes in the wave-local FFT
+// range [E, 64 E] (E = 8 in FP64, 16 in FP32), so their passes can write the
+// tiled row layout.
+static bool rows_path(const GridDev& g, int prec) {
+  const int Ew = prec ? 8 : 16;
+  auto wave_ok = [&](int L) { return L >= Ew && L <= 64 * Ew; };
+  return g.N >= 2 && g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
+         (g.N == 2 || wave_ok(g.n[2]));
+}
+
 static int ilog2(int64_t v) {
   int l = 0;
   while ((1ll << l) < v) ++l;
@@ -313,12 +324,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     // axis 2 (and 3) inverse passes write the tiled row layout the row kernel
     // reads; in 3-D the axis-2 result goes through a second buffer
     const int64_t per_rung = 2 * (int64_t)B * g.fsY;  // tiled rows (>= H1 * Q per field)
-    const int Ew = sizeof(T) == 8 ? 8 : 16;  // wave-local FFT range [E, 64 E]
-    auto wave_ok = [&](int L) { return L >= Ew && L <= 64 * Ew; };
-    // the row kernel needs the axes before it in the wave-local range (their
-    // passes write the tiled row layout)
-    const bool rows = g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
-                      (N == 2 || wave_ok(g.n[2]));
+    const bool rows = g.pack != 0;  // the power-of-two row path (rows_path)
     const bool rows3 = N == 3 && rows;
     const bool rows_gen = sizeof(T) == 8 && ladder_gen_rows_ok(g);
     const int64_t cap = d.Y_elems / y_units(pl);
@@ -667,14 +673,16 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     // writing it holds (grid axis 2 in 2-D, axis 3 in 3-D; 256 lanes x E values)
     const int E = pl->prec ? 8 : 16;
     const int L = ndims == 3 ? g.n[2] : g.n[1];
+    g.pack = rows_path(g, pl->prec) ? 1 : 0;
+    g.Hc = g.pack ? (int)(n_per_axis[0] / 2) : g.H1;
     int hp = 1;
-    while (hp < g.H1) hp <<= 1;
+    while (hp < g.Hc) hp <<= 1;
     g.TKy = ndims >= 2 ? std::min(hp, std::max(1, 256 * E / L)) : 1;
     // axes that are not powers of two: 8-column tiles (128-byte FP64 rows) of
     // the mixed-radix 2-D ladder
     if (!g.pow2 && ndims >= 2) g.TKy = 8;
     g.log2TKy = ilog2(g.TKy);
-    g.ktY = (g.H1 + g.TKy - 1) / g.TKy;
+    g.ktY = (g.Hc + g.TKy - 1) / g.TKy;
     g.fsY = (int64_t)g.ktY * g.Q * g.TKy;
   }
   pl->fixed = p.deterministic_binning ? 1 : 0;

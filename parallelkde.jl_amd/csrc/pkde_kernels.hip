@@ -312,9 +312,10 @@ struct AxisArgs {
   // row index of the tiled store: q' = o * oO + e * oE
   // (o: outer pencil index, e: position along the axis)
   int64_t oO, oE;
-  // ladder pass: TJ outer pencils per block, store mode 0 [k1][q], 1 [q][k1],
-  // 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass)
-  int TJ, log2TJ, log2otiles, mode;
+  // ladder pass: TJ outer pencils per block, store mode 0 [k1][q], 1 tiled
+  // rows, 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass); pack:
+  // column 0 carries X(k1 = 0) + i X(k1 = H1 columns), H1 = n1 / 2
+  int TJ, log2TJ, log2otiles, mode, pack;
   const cplx<T>* tw;
   // mixed-radix plan and twiddle table of the axis (GEN kernels)
   FftFactors fac;
@@ -619,6 +620,18 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
     for (int u = 0; u < E / R1; ++u)
 #pragma unroll
       for (int r = 0; r < R1; ++r) x[u * R1 + r] = col[t + u * tpp + r * nb1];
+    if (a.pack && k0 + kk == 0) {
+      // X(0) + i X(n1/2): the Nyquist column shares column 0's pencil
+      const cplx<T>* colN = src + (int64_t)a.H1 * a.Q + (o0 + jj) * L;
+#pragma unroll
+      for (int u = 0; u < E / R1; ++u)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const cplx<T> h = colN[t + u * tpp + r * nb1];
+          cplx<T>& v = x[u * R1 + r];
+          v = {v.re - h.im, v.im + h.re};
+        }
+    }
   }
   cplx<T>* pb = buf + (act ? p : 0) * PL;
   const int log2np = log2TK + log2TJ;
@@ -728,8 +741,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.dst = dst;
   a.src_rs = 0;
   a.dst_rs = dst_rung_stride;
-  a.fsize = (int64_t)g.H1 * g.Q;
-  a.dst_fsize = mode == 1 ? g.fsY : a.fsize;
+  a.fsize = (int64_t)g.H1 * g.Q;  // the spectra: H1 columns per field
   a.fid0 = 0;
   a.nf = nf;
   a.src_fid0 = 0;
@@ -742,18 +754,22 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.ksum = g.ksum;
   a.goff = g.koff[1];
   a.Q = g.Q;
-  a.H1 = g.H1;
+  // columns carried: H1, or n1/2 with the Nyquist column packed into column 0;
+  // the output holds only the carried columns (mode 1: tiled rows, fsY)
+  a.pack = mode != 0 && g.pack;
+  a.H1 = a.pack ? g.Hc : g.H1;
+  a.dst_fsize = mode == 1 ? g.fsY : (int64_t)a.H1 * g.Q;
   a.I = 1;
   a.O = g.n[2];
   a.TI = 1;
   a.mode = mode;
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
-  while (hp < g.H1) hp <<= 1;
+  while (hp < a.H1) hp <<= 1;
   a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(8, per_block)) : 1;
   a.TK = std::min(hp, per_block / a.TJ);
   if (mode == 1 && a.TK != g.TKy) return false;  // tile width fixed by the layout
-  a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+  a.ktiles = (a.H1 + a.TK - 1) / a.TK;
   a.itiles = 1;
   a.log2TI = 0;
   a.log2TK = ilog2c(a.TK);
@@ -789,7 +805,7 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.dst = dst;
   a.src_rs = src_rung_stride;
   a.dst_rs = dst_rung_stride;
-  a.fsize = (int64_t)g.H1 * g.Q;
+  a.fsize = (int64_t)g.Hc * g.Q;  // fields of the first pass's output
   a.dst_fsize = g.fsY;
   a.fid0 = 0;
   a.nf = nf;
@@ -803,7 +819,7 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.ksum = g.ksum;
   a.goff = g.koff[2];
   a.Q = g.Q;
-  a.H1 = g.H1;
+  a.H1 = g.Hc;  // the first pass packed the Nyquist column when g.pack
   // input: pencil (k1, j2) at k1 * Q + j2 * n3, contiguous along j3
   a.I = 1;
   a.O = g.n[1];
@@ -813,10 +829,10 @@ bool launch_axis3_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.oE = 1;
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
-  while (hp < g.H1) hp <<= 1;
+  while (hp < g.Hc) hp <<= 1;
   a.TK = std::min(hp, per_block);
   if (a.TK != g.TKy) return false;  // tile width fixed by the layout
-  a.ktiles = (g.H1 + a.TK - 1) / a.TK;
+  a.ktiles = (g.Hc + a.TK - 1) / a.TK;
   a.itiles = 1;
   a.log2TI = 0;
   a.log2TK = ilog2c(a.TK);
@@ -1033,6 +1049,7 @@ struct VmrArgs {
   // rows3: rows are q' = j3 + n3 * j2 (3-D), grid row q = j2 + n2 * j3
   int log2TKy, rows3, log2n2, log2n3;
   int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
+  int pack;      // column 0 of the chunk holds X(0) + i X(n1/2) (row kernel)
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -1254,7 +1271,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   // Per lane: PPT (row, k1) items of the half spectrum k1 < n1/2, plus the
   // Nyquist term k1 = n1/2 of row `lane` for lanes < 2W.  Row index
   // ty * W + w: ty = 0 sums field, ty = 1 squares field.
-  T ld_g[PPT];
+  T ld_g[PPT], ld_gN[PPT];
   int64_t ld_off[PPT];
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
@@ -1262,14 +1279,17 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     const int k1 = id & (H - 1);
     const int rw = id >> (LOG2N1 - 1);
     ld_g[u] = (rw >> LOG2W) ? gv1[k1] : gm1[k1];
+    ld_gN[u] = (rw >> LOG2W) ? gv1[H] : gm1[H];  // Nyquist factor (packed column 0)
     // squares rows sit B fields after the sums rows
     ld_off[u] = toff(rw & (W - 1), k1) + ((rw >> LOG2W) ? (int64_t)a.B * a.fsize : 0);
   }
-  const bool nyq = lane < 2 * W;
+  // Nyquist column k1 = n1/2: its own tile, or packed into column 0
+  const bool pack = a.pack != 0;
+  const bool nyq = !pack && lane < 2 * W;
   const int nrow = nyq ? lane : 0;
   const int nty = nrow >> LOG2W, nw = nrow & (W - 1);
   const T ng = nty ? gv1[H] : gm1[H];
-  const int64_t nyq_off = toff(nw, H) + (nty ? (int64_t)a.B * a.fsize : 0);
+  const int64_t nyq_off = pack ? 0 : toff(nw, H) + (nty ? (int64_t)a.B * a.fsize : 0);
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -1306,11 +1326,19 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       const int id = lane + 64 * u;
       const int k1 = id & (H - 1);
       const int rw = id >> (LOG2N1 - 1);
-      const cplx<T> A = cscale(pa[u], ld_g[u]);
-      const cplx<T> Bv = has2 ? cscale(pb2[u], ld_g[u]) : zero;
       cplx<T>* row = Z + rw * PL;
-      row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
-      if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      if (pack && k1 == 0) {
+        // packed column: real part k1 = 0, imaginary part k1 = n1/2 (both
+        // real-valued per field); A + i B for the pair at each
+        const T br = has2 ? pb2[u].re : (T)0, bi = has2 ? pb2[u].im : (T)0;
+        row[pad8(0)] = {pa[u].re * ld_g[u], br * ld_g[u]};
+        row[pad8(H)] = {pa[u].im * ld_gN[u], bi * ld_gN[u]};
+      } else {
+        const cplx<T> A = cscale(pa[u], ld_g[u]);
+        const cplx<T> Bv = has2 ? cscale(pb2[u], ld_g[u]) : zero;
+        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      }
     }
     if (nyq) {
       const cplx<T> A = cscale(pna, ng);
@@ -1406,6 +1434,7 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   a.fsize = g.fsY;
   a.log2TKy = g.log2TKy;
   a.tstride = g.Q << g.log2TKy;
+  a.pack = g.pack;
   a.rows3 = g.N == 3;
   a.log2n2 = g.log2n[1];
   a.log2n3 = g.log2n[2];

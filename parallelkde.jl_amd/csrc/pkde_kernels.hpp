@@ -24,6 +24,11 @@ struct GridDev {
   // contiguous tile.  Rows are q' = q in 2-D and q' = j3 + n3 * j2 in 3-D.
   int TKy, log2TKy, ktY;
   int64_t fsY;
+  // Columns k1 a rung chunk carries: H1, or n1/2 when `pack` -- the
+  // power-of-two row path carries the k1 = 0 and k1 = n1/2 columns of a real
+  // field as one complex pencil X0 + i X(n1/2) (both stay real through the
+  // inverse passes along the other axes), which removes the Nyquist tile.
+  int Hc, pack;
   int64_t Q;   // n2 * n3 (pencils per axis-1 row set)
   double lo[kMaxDims], hi[kMaxDims], h[kMaxDims];
   int koff[kMaxDims];  // offset of axis i in per-rung Gaussian tables
