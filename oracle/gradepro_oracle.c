@@ -208,6 +208,9 @@ int og_check_data(const og_grid *g, const double *samples, int64_t n) {
 }
 
 /* generate_dirac_cpu! (DirectSpace.jl:196-242) for one bootstrap column. */
+static int og_deposit_reversed = 0;
+void og_set_deposit_order(int reversed) { og_deposit_reversed = reversed ? 1 : 0; }
+
 static void og_dirac_one(const og_grid *g, const double *samples, int64_t n,
                          const int32_t *col, double *sums, double *squares) {
   const int N = g->ndims;
@@ -219,7 +222,8 @@ static void og_dirac_one(const og_grid *g, const double *samples, int64_t n,
   stride[0] = 1;
   for (int i = 1; i < N; ++i) stride[i] = stride[i - 1] * g->n[i - 1];
 
-  for (int64_t s = 0; s < n; ++s) {
+  for (int64_t s0 = 0; s0 < n; ++s0) {
+    const int64_t s = og_deposit_reversed ? n - 1 - s0 : s0;
     const double *x = samples + (int64_t)col[s] * N;
     int64_t il[OG_MAX_DIMS];
     double rl[OG_MAX_DIMS], rh[OG_MAX_DIMS];

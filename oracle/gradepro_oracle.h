@@ -101,6 +101,11 @@ int og_fft_nd(int ndims, const int64_t *n, double *data, int sign);
  * rounding envelope; mode 2 also perturbs lengths whose factors are all 4).
  * Not thread-safe: set it between estimates. */
 void og_set_fft_order(int mode);
+/* 1: the Dirac-sequence deposits of each resample are summed from the last
+ * draw to the first (0: in draw order, as the reference does).  The same sums
+ * in exact arithmetic; a second rounding-envelope probe, for comparisons with
+ * binning that sums in another order (FP64 atomics).  Not thread-safe. */
+void og_set_deposit_order(int reversed);
 /* propagate_time_cpu! (FourierSpace.jl:215-262) on one complex field. vars may
  * be NULL (means-only form). */
 void og_propagate(const og_grid *g, const double *kgrid_axes, const double *t,

@@ -1149,6 +1149,150 @@ __global__ __launch_bounds__(kThreads) void k_ladder_vmr(VmrArgs<T> a) {
   }
 }
 
+// 1-D ladder for power-of-two grids in the wave-local FFT range: one block
+// of kWaves1d wavefronts per rung.  Each round, wavefront w takes bootstrap
+// pair (b, b + 1), b = 2 (kWaves1d round + w): it loads the sums and squares
+// fields of both, builds and transforms the two complex pencils and leaves
+// the two VMR ratios of every point in its LDS slice; after a barrier each
+// thread applies the Welford updates of its points over the round's
+// bootstraps in order.  Same arithmetic as k_ladder_vmr.
+constexpr int kWaves1d = 8;
+
+template <typename T, int LOG2N1>
+__global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
+  constexpr int n1 = 1 << LOG2N1;
+  constexpr int H = n1 / 2;
+  constexpr int E = sizeof(T) == 8 ? 8 : 16;
+  constexpr int PL = pencil_stride(n1);
+  constexpr int lp = n1 / E;      // lanes per pencil
+  constexpr int ppc = 64 / lp;    // pencils per wave-wide FFT call
+  constexpr int NT = 64 * kWaves1d;
+  constexpr int PPT = (n1 + NT - 1) / NT;  // points per thread in the Welford phase
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cplx<T>* twl = (cplx<T>*)smem + kWaves1d * 2 * PL;
+  for (int k = tid; k < n1; k += NT) twl[k] = a.tw[n1 - 2 + k];
+  __syncthreads();
+  cplx<T>* Z = (cplx<T>*)smem + wv * 2 * PL;  // sums pencil; squares pencil at Z + PL
+  const int r = a.r0 + (int)blockIdx.x;
+  const T* gm1 = a.gM + (int64_t)r * a.ksum;
+  const T* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<T>* src = a.src + (int64_t)blockIdx.x * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  const int B = a.B;
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+  const cplx<T> zero = {(T)0, (T)0};
+  for (int b0 = 0; b0 < B; b0 += 2 * kWaves1d) {
+    const int b = b0 + 2 * wv;
+    if (b < B) {
+      const bool has2 = b + 1 < B;
+      for (int id = lane; id < 2 * (H + 1); id += 64) {
+        const int ty = id > H;
+        const int k1 = ty ? id - H - 1 : id;
+        const int fa = ty * B + b;
+        const T gk = ty ? gv1[k1] : gm1[k1];
+        const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + k1], gk);
+        const cplx<T> Bv = has2 ? cscale(src[(int64_t)(fa + 1) * a.fsize + k1], gk) : zero;
+        cplx<T>* row = Z + ty * PL;
+        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        if (k1 > 0 && k1 < H) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      }
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < 2; c += ppc) {
+        const int p = c + lane / lp;
+        const bool act = p < 2;
+        wave_fft<LOG2N1, 1, T, E>(Z + (act ? p : 0) * PL, lane % lp, act, twl);
+      }
+      // the ratios of bootstraps b and b + 1 replace the pencils (as doubles
+      // in the first 8 bytes of each slot; every lane rewrites what it read)
+      for (int j = lane; j < n1; j += 64) {
+        const cplx<T> zm = Z[pad8(j)];
+        const cplx<T> zv = Z[PL + pad8(j)];
+        const double x0 = vmr_ratio(fabs((double)zv.re) * vfac, fabs((double)zm.re) * a.invG, a.invn);
+        const double x1 =
+            has2 ? vmr_ratio(fabs((double)zv.im) * vfac, fabs((double)zm.im) * a.invG, a.invn) : 0.0;
+        *(double*)(Z + pad8(j)) = x0;
+        *(double*)(Z + PL + pad8(j)) = x1;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int j = tid + u * NT;
+      if (j < n1) {
+        for (int w = 0; w < kWaves1d; ++w) {
+          const int bw = b0 + 2 * w;
+          if (bw >= B) break;
+          const cplx<T>* Zw = (const cplx<T>*)smem + w * 2 * PL;
+          {
+            const double xv = *(const double*)(Zw + pad8(j));
+            const double d = xv - mean[u];
+            mean[u] += d / (double)(bw + 1);
+            m2[u] += d * (xv - mean[u]);
+          }
+          if (bw + 1 < B) {
+            const double xv = *(const double*)(Zw + PL + pad8(j));
+            const double d = xv - mean[u];
+            mean[u] += d / (double)(bw + 2);
+            m2[u] += d * (xv - mean[u]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int j = tid + u * NT;
+    if (j < n1) {
+      const double vv = sc * m2[u] / (double)(B - 1);
+      a.vmr[(int64_t)r * a.G + j] = isfinite(vv) ? log(vv) : (double)NAN;
+    }
+  }
+}
+
+template <typename T, int LOG2N1>
+static void ladder_1d_launch(unsigned blocks, hipStream_t s, const VmrArgs<T>& a) {
+  constexpr int n1 = 1 << LOG2N1;
+  const size_t lds = ((size_t)kWaves1d * 2 * pencil_stride(n1) + n1) * sizeof(cplx<T>);
+  static const bool attr = [lds] {
+    // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
+    (void)hipFuncSetAttribute((const void*)k_ladder_1d<T, LOG2N1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return true;
+  }();
+  (void)attr;
+  k_ladder_1d<T, LOG2N1><<<blocks, 64 * kWaves1d, lds, s>>>(a);
+}
+
+// True if the 1-D ladder took the wave-parallel form (n1 a power of two in
+// the wave-local FFT range, at least 16).
+template <typename T>
+static bool ladder_1d_dispatch(int log2n1, unsigned blocks, hipStream_t s, const VmrArgs<T>& a) {
+  switch (log2n1) {
+    case 4: ladder_1d_launch<T, 4>(blocks, s, a); return true;
+    case 5: ladder_1d_launch<T, 5>(blocks, s, a); return true;
+    case 6: ladder_1d_launch<T, 6>(blocks, s, a); return true;
+    case 7: ladder_1d_launch<T, 7>(blocks, s, a); return true;
+    case 8: ladder_1d_launch<T, 8>(blocks, s, a); return true;
+    case 9: ladder_1d_launch<T, 9>(blocks, s, a); return true;
+    case 10:
+      if constexpr (sizeof(T) == 4) {
+        ladder_1d_launch<T, 10>(blocks, s, a);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
 template <typename T, bool GEN>
 static void ladder_vmr_dispatch(int ppt, unsigned blocks, size_t lds, hipStream_t s,
                                 const VmrArgs<T>& a) {
@@ -1206,6 +1350,7 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.fac = g.fac[0];
   a.twg = twg ? (const cplx<T>*)twg + g.koff[0] : nullptr;
   const bool gen = (n1 & (n1 - 1)) != 0;
+  if (g.N == 1 && !gen && ladder_1d_dispatch<T>(g.log2n[0], (unsigned)R, s, a)) return;
   const unsigned blocks = (unsigned)((int64_t)R * a.ntiles);
   const size_t lds = (size_t)2 * NBP * TQ * n1 * sizeof(cplx<T>);
   if (gen)

@@ -83,6 +83,7 @@ def lib():
                                          _dp, C.c_void_p, C.c_int]
         L.og_fft_nd.argtypes = [C.c_int, C.POINTER(C.c_int64), _dp, C.c_int]
         L.og_set_fft_order.argtypes = [C.c_int]
+        L.og_set_deposit_order.argtypes = [C.c_int]
         L.og_propagate.argtypes = [C.POINTER(OgGrid), _dp, _dp, _dp, _dp, C.c_void_p, _dp, C.c_void_p]
         L.og_scaled_vmr.argtypes = [C.c_int, C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, C.c_int64, _dp]
         L.og_full_means.argtypes = [C.c_int64, _dp, _dp]
@@ -174,19 +175,25 @@ def set_fft_order(mode: int):
     lib().og_set_fft_order(int(mode))
 
 
-def rounding_envelope(g: OgGrid, samples_col, idx_bs, **kw):
-    """The oracle's estimate under its default FFT radix order and under the
-    two other summation orders: how far FP64 rounding alone moves the
-    reference's result on this input.  Returns (default result, [results
-    under the other orders])."""
+def rounding_envelope(g: OgGrid, samples_col, idx_bs, deposits=False, **kw):
+    """The oracle's estimate in its default summation orders and in others:
+    the FFT radices reversed, radix-2 passes, and (deposits=True) each
+    resample's deposits summed last draw first -- how far FP64 rounding alone
+    moves the reference's result on this input.  Returns (default result,
+    [results under the other orders])."""
     r0 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
     others = []
     try:
         for mode in (1, 2):
             set_fft_order(mode)
             others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
+        set_fft_order(0)
+        if deposits:
+            lib().og_set_deposit_order(1)
+            others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
     finally:
         set_fft_order(0)
+        lib().og_set_deposit_order(0)
     return r0, others
 
 

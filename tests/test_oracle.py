@@ -81,6 +81,24 @@ def test_fft_other_radix_orders_are_the_same_dft(n, mode):
         assert not np.array_equal(f, O.fft_nd(a, -1))
 
 
+def test_reversed_deposit_order_is_the_same_binning():
+    """The reversed deposit order (a rounding-envelope probe) sums the same
+    terms: equal to the default order within FP64 rounding, same support."""
+    rng = np.random.default_rng(11)
+    g = O.make_grid([-3.0, -3.0], [3.0, 3.0], [33, 29])
+    x = rng.uniform(-2.9, 2.9, size=(4000, 2))
+    idx = rng.integers(0, 4000, size=(3, 4000)).astype(np.int32)
+    _, s0, q0 = O.dirac_sequences(g, x, idx)
+    O.lib().og_set_deposit_order(1)
+    try:
+        _, s1, q1 = O.dirac_sequences(g, x, idx)
+    finally:
+        O.lib().og_set_deposit_order(0)
+    assert np.allclose(s1, s0, rtol=1e-13, atol=0) and np.allclose(q1, q0, rtol=1e-13, atol=0)
+    assert np.array_equal(s1 > 0, s0 > 0)
+    assert not np.array_equal(s1, s0)  # a different rounding somewhere
+
+
 def test_fft_round_trip_and_parseval():
     rng = np.random.default_rng(3)
     a = rng.standard_normal((6, 10, 9)) + 1j * rng.standard_normal((6, 10, 9))
