@@ -1258,6 +1258,106 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
   }
 }
 
+// The same for lengths that are not powers of two but fit the wave-local
+// mixed-radix FFT (a.fac: radices 2, 3, 4, 5, 8; n1 <= 512): PPT = points per
+// thread of the Welford phase.
+template <typename T>
+__global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d_gen(VmrArgs<T> a) {
+  constexpr int NT = 64 * kWaves1d;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = a.n1, H1 = a.H1;
+  const int PL = gen_pencil_stride(n1);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cplx<T>* twl = (cplx<T>*)smem + kWaves1d * 2 * PL;
+  for (int k = tid; k < n1; k += NT) twl[k] = a.twg[k];
+  __syncthreads();
+  cplx<T>* Z = (cplx<T>*)smem + wv * 2 * PL;
+  const int r = a.r0 + (int)blockIdx.x;
+  const T* gm1 = a.gM + (int64_t)r * a.ksum;
+  const T* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<T>* src = a.src + (int64_t)blockIdx.x * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  const int B = a.B;
+  const int j = tid;  // n1 <= 512 = NT: one point per thread
+  double mean = 0.0, m2 = 0.0;
+  const cplx<T> zero = {(T)0, (T)0};
+  for (int b0 = 0; b0 < B; b0 += 2 * kWaves1d) {
+    const int b = b0 + 2 * wv;
+    if (b < B) {
+      const bool has2 = b + 1 < B;
+      for (int id = lane; id < 2 * H1; id += 64) {
+        const int ty = id >= H1;
+        const int k1 = ty ? id - H1 : id;
+        const int fa = ty * B + b;
+        const T gk = ty ? gv1[k1] : gm1[k1];
+        const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + k1], gk);
+        const cplx<T> Bv = has2 ? cscale(src[(int64_t)(fa + 1) * a.fsize + k1], gk) : zero;
+        cplx<T>* row = Z + ty * PL;
+        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        if (k1 > 0 && 2 * k1 < n1) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      }
+      wave_sync();
+      wave_fft_gen<1, T, 64>(Z, lane, true, a.fac, twl);
+      wave_fft_gen<1, T, 64>(Z + PL, lane, true, a.fac, twl);
+      for (int jj = lane; jj < n1; jj += 64) {
+        const cplx<T> zm = Z[pad8(jj)];
+        const cplx<T> zv = Z[PL + pad8(jj)];
+        const double x0 = vmr_ratio(fabs((double)zv.re) * vfac, fabs((double)zm.re) * a.invG, a.invn);
+        const double x1 =
+            has2 ? vmr_ratio(fabs((double)zv.im) * vfac, fabs((double)zm.im) * a.invG, a.invn) : 0.0;
+        *(double*)(Z + pad8(jj)) = x0;
+        *(double*)(Z + PL + pad8(jj)) = x1;
+      }
+    }
+    __syncthreads();
+    if (j < n1) {
+      for (int w = 0; w < kWaves1d; ++w) {
+        const int bw = b0 + 2 * w;
+        if (bw >= B) break;
+        const cplx<T>* Zw = (const cplx<T>*)smem + w * 2 * PL;
+        {
+          const double xv = *(const double*)(Zw + pad8(j));
+          const double d = xv - mean;
+          mean += d / (double)(bw + 1);
+          m2 += d * (xv - mean);
+        }
+        if (bw + 1 < B) {
+          const double xv = *(const double*)(Zw + PL + pad8(j));
+          const double d = xv - mean;
+          mean += d / (double)(bw + 2);
+          m2 += d * (xv - mean);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (j < n1) {
+    const double vv = a.vscale[r] * m2 / (double)(B - 1);
+    a.vmr[(int64_t)r * a.G + j] = isfinite(vv) ? log(vv) : (double)NAN;
+  }
+}
+
+template <typename T>
+static bool ladder_1d_gen_launch(const GridDev& g, unsigned blocks, hipStream_t s,
+                                 const VmrArgs<T>& a) {
+  const int n1 = g.n[0];
+  if (n1 > 64 * kWaves1d || !wave_gen_fits(g.facw[0]) || !a.twg) return false;
+  const size_t lds = ((size_t)kWaves1d * 2 * gen_pencil_stride(n1) + n1) * sizeof(cplx<T>);
+  static const bool attr = [] {
+    // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
+    (void)hipFuncSetAttribute((const void*)k_ladder_1d_gen<T>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(((size_t)kWaves1d * 2 * gen_pencil_stride(512) + 512) *
+                                    sizeof(cplx<T>)));
+    return true;
+  }();
+  (void)attr;
+  VmrArgs<T> b = a;
+  b.fac = g.facw[0];
+  k_ladder_1d_gen<T><<<blocks, 64 * kWaves1d, lds, s>>>(b);
+  return true;
+}
+
 template <typename T, int LOG2N1>
 static void ladder_1d_launch(unsigned blocks, hipStream_t s, const VmrArgs<T>& a) {
   constexpr int n1 = 1 << LOG2N1;
@@ -1351,6 +1451,7 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
   a.twg = twg ? (const cplx<T>*)twg + g.koff[0] : nullptr;
   const bool gen = (n1 & (n1 - 1)) != 0;
   if (g.N == 1 && !gen && ladder_1d_dispatch<T>(g.log2n[0], (unsigned)R, s, a)) return;
+  if (g.N == 1 && gen && ladder_1d_gen_launch<T>(g, (unsigned)R, s, a)) return;
   const unsigned blocks = (unsigned)((int64_t)R * a.ntiles);
   const size_t lds = (size_t)2 * NBP * TQ * n1 * sizeof(cplx<T>);
   if (gen)
