@@ -109,8 +109,8 @@ int pkde_find_grid(pkde_ctx *ctx, int ndims, const double *d_samples, int64_t n,
 
 /* A plan binds a grid (Grid: Grids.jl:67-87 -- per-axis lo, hi, spacing and
  * point count) and the estimator keywords.  Axis sizes must lie in [2, 4096]
- * and have no prime factor above 13 (the radices of the mixed-radix FFT;
- * others return PKDE_UNSUPPORTED); ndims in 1..3 (the reference's GPU path is capped at 3-D,
+ * (any factorization: prime factors above 13 take direct-DFT passes; other
+ * sizes return PKDE_UNSUPPORTED); ndims in 1..3 (the reference's GPU path is capped at 3-D,
  * GradeProEstimator.jl:811-814).  Replaces initialize_estimator
  * (GradeProEstimator.jl:700-735) for the :rocm device. */
 int pkde_gradepro_plan_create(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,

@@ -371,7 +371,9 @@ static void og_fft1_exec(const og_fft1 *p, double *x, double *work, int sign) {
         const int64_t wstep = n / R;
         for (int64_t q = 0; q < R; ++q) {
           double sr = 0.0, si = 0.0;
-          for (int64_t r = 0; r < R; ++r) {
+          for (int64_t rr = 0; rr < R; ++rr) {
+            /* mode 1 also sums the direct DFT's terms last first */
+            const int64_t r = og_fft_mode == 1 ? R - 1 - rr : rr;
             double c, s;
             og_tw(p, ((r * q) % R) * wstep, sign, &c, &s);
             sr += ar[r] * c - ai[r] * s;

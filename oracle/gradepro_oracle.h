@@ -94,7 +94,8 @@ int og_dirac_sequences(const og_grid *g, const double *samples, int64_t n,
 int og_fft_nd(int ndims, const int64_t *n, double *data, int sign);
 /* Selects the radix order of og_fft_nd for the calling process: 0 (default)
  * applies radix 4, then 2, then odd primes ascending; 1 applies the same
- * factors in reverse order; 2 splits the power of two into radix-2 passes
+ * factors in reverse order and sums each direct prime-radix DFT from its last
+ * term to its first; 2 splits the power of two into radix-2 passes
  * (then odd primes ascending).  All are exact DFTs in exact arithmetic; the
  * others sum in a different order, so running a whole estimate under each
  * measures how far FP64 rounding alone moves the reference's result (its

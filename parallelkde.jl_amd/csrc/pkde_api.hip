@@ -49,7 +49,8 @@ static void dfree(void* p) {
 
 static bool is_pow2(int64_t v) { return v >= 2 && (v & (v - 1)) == 0; }
 
-// Radix list of a 13-smooth length: 16s, one 8/4/2, then 3, 5, 7, 11, 13.
+// Radix list of a length: 16s, one 8/4/2, then 3, 5, 7, 11, 13, then any
+// larger prime factors (direct-DFT passes).  False only past 16 passes.
 static bool factorize(int L, FftFactors& f) {
   f.L = L;
   f.np = 0;
@@ -60,6 +61,10 @@ static bool factorize(int L, FftFactors& f) {
   else if (n % 2 == 0) f.rad[f.np++] = 2, n /= 2;
   for (int p : {3, 5, 7, 11, 13})
     while (n % p == 0 && f.np < 16) f.rad[f.np++] = p, n /= p;
+  // larger primes: one direct pass each (lds_fft_gen)
+  for (int p = 17; p * p <= n; p += 2)
+    while (n % p == 0 && f.np < 16) f.rad[f.np++] = p, n /= p;
+  if (n > 1 && f.np < 16) f.rad[f.np++] = n, n = 1;
   return n == 1;
 }
 
@@ -364,10 +369,11 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
                                        s))
           return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
       }
-      if (rows)
-        launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
-                              d.vmr, s);
-      else
+      if (rows) {
+        if (!launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
+                                   d.vmr, s))
+          return fail(PKDE_DEVICE_ERROR, "row kernel layout mismatch");
+      } else
         launch_ladder_vmr<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, twg, d.vmr,
                              s);
     }
@@ -647,7 +653,7 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     if (n_per_axis[i] < 2 || n_per_axis[i] > 4096 || !factorize((int)n_per_axis[i], g.fac[i])) {
       delete pl;
       return fail(PKDE_UNSUPPORTED,
-                  "grid axis sizes must be in [2, 4096] with no prime factor above 13");
+                  "grid axis sizes must be in [2, 4096]");
     }
     if (!is_pow2(n_per_axis[i])) g.pow2 = 0;
     factorize_wave((int)n_per_axis[i], g.facw[i]);

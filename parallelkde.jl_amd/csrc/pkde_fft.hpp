@@ -181,8 +181,8 @@ __device__ void lds_fft(cplx<T>* buf, int log2L, int npen, int pitch,
 }
 
 // ------------------------------------------------------------ mixed radix
-// Pencil length L = prod(rad[0..np)) with radices in {2, 3, 4, 5, 7, 8, 11,
-// 13, 16} (13-smooth lengths); tw[k] = exp(-2 pi i k / L), k < L.
+// Pencil length L = prod(rad[0..np)) with radices 2, 3, 4, 5, 7, 8, 11, 13,
+// 16 or any larger prime; tw[k] = exp(-2 pi i k / L), k < L.
 struct FftFactors {
   int L, np;
   int rad[16];
@@ -264,10 +264,60 @@ __device__ __forceinline__ void stockham_pass_gen(cplx<T>* buf, int L, int npen,
   __syncthreads();
 }
 
-// Batched in-LDS FFT of any 13-smooth length f.L <= 4096: natural order in
-// and out, unnormalised; DIR = -1 forward, +1 inverse.  Pencils are done in
-// groups of at most 4096 points so every pass fits one round of 256 threads.
-// Must be called by the whole workgroup of 256 threads.
+// Stockham pass of a prime radix R above 13, known only at run time: each
+// output is formed directly, y = sum_r x_r exp(DIR 2 pi i r (jm + s Ns) /
+// (Ns R)) over the butterfly's R inputs (the pass twiddle and the DFT root
+// folded into one table entry, index r (jm + s Ns) L / (Ns R) mod L).  A
+// thread forms at most 16 outputs (npen * L <= 4096 with 256 threads), all
+// of them before the barrier, so the pass is in place.  O(L R) work per
+// pencil: the axis lengths that need it are small.
+template <int DIR, typename T>
+__device__ __forceinline__ void stockham_pass_any(cplx<T>* buf, int L, int npen, int pitch, int Ns,
+                                                  int R, const cplx<T>* __restrict__ tw, int tid,
+                                                  int nthr) {
+  constexpr int OPT = 16;
+  const int nb = L / R;
+  const int span = Ns * R;
+  const int twstep = L / span;
+  const int total = npen * L;
+  cplx<T> y[OPT];
+  int dst[OPT];
+#pragma unroll
+  for (int k = 0; k < OPT; ++k) {
+    const int o = tid + k * nthr;
+    dst[k] = -1;
+    if (o < total) {
+      const int pen = o / L, po = o - pen * L;
+      const int q = po / span, rem = po - q * span;
+      const int s = rem / Ns, jm = rem - s * Ns;
+      const int j = q * Ns + jm;
+      const cplx<T>* pb = buf + (int64_t)pen * pitch;
+      const int64_t dk = (int64_t)(jm + s * Ns) * twstep;  // < L
+      cplx<T> acc = pb[j];
+      int64_t e = 0;
+      for (int r = 1; r < R; ++r) {
+        e += dk;
+        if (e >= L) e -= L;
+        cplx<T> f = tw[e];
+        if (DIR > 0) f.im = -f.im;
+        acc = cadd(acc, cmul(pb[j + r * nb], f));
+      }
+      y[k] = acc;
+      dst[k] = pen * pitch + po;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < OPT; ++k)
+    if (dst[k] >= 0) buf[dst[k]] = y[k];
+  __syncthreads();
+}
+
+// Batched in-LDS FFT of any length f.L <= 4096: natural order in and out,
+// unnormalised; DIR = -1 forward, +1 inverse.  Radices 2-16 have fixed
+// butterflies; a prime factor above 13 takes the direct pass above.  Pencils
+// are done in groups of at most 4096 points so every pass fits one round of
+// 256 threads.  Must be called by the whole workgroup of 256 threads.
 template <int DIR, typename T>
 __device__ void lds_fft_gen(cplx<T>* buf, const FftFactors& f, int npen, int pitch,
                             const cplx<T>* __restrict__ tw, int tid, int nthr) {
@@ -287,7 +337,8 @@ __device__ void lds_fft_gen(cplx<T>* buf, const FftFactors& f, int npen, int pit
         case 8: stockham_pass_gen<8, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
         case 11: stockham_pass_gen<11, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
         case 13: stockham_pass_gen<13, 2, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
-        default: stockham_pass_gen<16, 1, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        case 16: stockham_pass_gen<16, 1, DIR, T>(b, L, cnt, pitch, Ns, tw, tid, nthr); break;
+        default: stockham_pass_any<DIR, T>(b, L, cnt, pitch, Ns, f.rad[p], tw, tid, nthr); break;
       }
       Ns *= f.rad[p];
     }

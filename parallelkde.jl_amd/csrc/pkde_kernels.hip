@@ -1049,7 +1049,6 @@ struct VmrArgs {
   // rows3: rows are q' = j3 + n3 * j2 (3-D), grid row q = j2 + n2 * j3
   int log2TKy, rows3, log2n2, log2n3;
   int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
-  int pack;      // column 0 of the chunk holds X(0) + i X(n1/2) (row kernel)
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -1514,10 +1513,11 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     return (int64_t)(k1 >> ltk) * a.tstride + ((q0 + row) << ltk) + (k1 & tkm);
   };
 
-  // Per lane: PPT (row, k1) items of the half spectrum k1 < n1/2, plus the
-  // Nyquist term k1 = n1/2 of row `lane` for lanes < 2W.  Row index
-  // ty * W + w: ty = 0 sums field, ty = 1 squares field.
-  T ld_g[PPT], ld_gN[PPT];
+  // Per lane: PPT (row, k1) items of the half spectrum k1 < n1/2.  Row index
+  // ty * W + w: ty = 0 sums field, ty = 1 squares field.  The Nyquist column
+  // k1 = n1/2 is packed into column 0 (imaginary part); its factors are the
+  // same for every lane of a field type.
+  T ld_g[PPT];
   int64_t ld_off[PPT];
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
@@ -1525,17 +1525,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     const int k1 = id & (H - 1);
     const int rw = id >> (LOG2N1 - 1);
     ld_g[u] = (rw >> LOG2W) ? gv1[k1] : gm1[k1];
-    ld_gN[u] = (rw >> LOG2W) ? gv1[H] : gm1[H];  // Nyquist factor (packed column 0)
     // squares rows sit B fields after the sums rows
     ld_off[u] = toff(rw & (W - 1), k1) + ((rw >> LOG2W) ? (int64_t)a.B * a.fsize : 0);
   }
-  // Nyquist column k1 = n1/2: its own tile, or packed into column 0
-  const bool pack = a.pack != 0;
-  const bool nyq = !pack && lane < 2 * W;
-  const int nrow = nyq ? lane : 0;
-  const int nty = nrow >> LOG2W, nw = nrow & (W - 1);
-  const T ng = nty ? gv1[H] : gm1[H];
-  const int64_t nyq_off = pack ? 0 : toff(nw, H) + (nty ? (int64_t)a.B * a.fsize : 0);
+  const T gNm = gm1[H], gNv = gv1[H];
 
   double mean[PPT], m2[PPT];
 #pragma unroll
@@ -1548,7 +1541,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   // blocks), issued one pair ahead so they are in flight during the current
   // pair's transform and Welford updates.  Branch-free: a pair without a
   // second field reloads the first and discards it.
-  cplx<T> pa[PPT], pb2[PPT], pna, pnb;
+  cplx<T> pa[PPT], pb2[PPT];
   double pi1, pi2;
   auto issue = [&](int b) {
     const cplx<T>* f = src + (int64_t)b * a.fsize;
@@ -1558,8 +1551,6 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       pa[u] = f[ld_off[u]];
       pb2[u] = f[ld_off[u] + o2];
     }
-    pna = f[nyq_off];
-    pnb = f[nyq_off + o2];
     pi1 = a.invk[b];
     pi2 = a.invk[b + 1 < B ? b + 1 : b];
   };
@@ -1573,23 +1564,19 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
       const int k1 = id & (H - 1);
       const int rw = id >> (LOG2N1 - 1);
       cplx<T>* row = Z + rw * PL;
-      if (pack && k1 == 0) {
+      if (k1 == 0) {
         // packed column: real part k1 = 0, imaginary part k1 = n1/2 (both
         // real-valued per field); A + i B for the pair at each
+        const T gN = (rw >> LOG2W) ? gNv : gNm;
         const T br = has2 ? pb2[u].re : (T)0, bi = has2 ? pb2[u].im : (T)0;
         row[pad8(0)] = {pa[u].re * ld_g[u], br * ld_g[u]};
-        row[pad8(H)] = {pa[u].im * ld_gN[u], bi * ld_gN[u]};
+        row[pad8(H)] = {pa[u].im * gN, bi * gN};
       } else {
         const cplx<T> A = cscale(pa[u], ld_g[u]);
         const cplx<T> Bv = has2 ? cscale(pb2[u], ld_g[u]) : zero;
         row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
         if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
       }
-    }
-    if (nyq) {
-      const cplx<T> A = cscale(pna, ng);
-      const cplx<T> Bv = has2 ? cscale(pnb, ng) : zero;
-      Z[nrow * PL + pad8(H)] = {A.re - Bv.im, A.im + Bv.re};
     }
     const double inv1 = pi1;
     const double inv2 = has2 ? pi2 : 0.0;
@@ -1666,10 +1653,11 @@ static void rows_dispatch(int log2n1, unsigned blocks, size_t lds, hipStream_t s
 }
 
 template <typename T>
-void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
                         const double* cV, const double* vscale, const T* tw, const double* invk,
                         double* vmr, hipStream_t s) {
+  if (!g.pack) return false;  // the kernel reads the packed Nyquist column
   VmrArgs<T> a;
   const int n1 = g.n[0];
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -1680,7 +1668,6 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
   a.fsize = g.fsY;
   a.log2TKy = g.log2TKy;
   a.tstride = g.Q << g.log2TKy;
-  a.pack = g.pack;
   a.rows3 = g.N == 3;
   a.log2n2 = g.log2n[1];
   a.log2n3 = g.log2n[2];
@@ -1713,13 +1700,14 @@ void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
     rows_dispatch<T, E, 4>(g.log2n[0], blocks, lds, s, a);
   else
     rows_dispatch<T, E, 8>(g.log2n[0], blocks, lds, s, a);
+  return true;
 }
 
-template void launch_ladder_rows<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t,
+template bool launch_ladder_rows<float>(const GridDev&, const cplx<float>*, int64_t, int, int64_t,
                                         int, int, const float*, const float*, const double*,
                                         const double*, const float*, const double*, double*,
                                         hipStream_t);
-template void launch_ladder_rows<double>(const GridDev&, const cplx<double>*, int64_t, int,
+template bool launch_ladder_rows<double>(const GridDev&, const cplx<double>*, int64_t, int,
                                          int64_t, int, int, const double*, const double*,
                                          const double*, const double*, const double*,
                                          const double*, double*, hipStream_t);

@@ -124,9 +124,10 @@ void launch_ladder_vmr(const GridDev& g, const cplx<T>* src, int64_t src_rung_st
                        double* vmr, hipStream_t s);
 
 // Row-per-wave form of the same ladder step for Q >= 64 rows and
-// 16 <= n1 <= 512, reading the tiled row layout (see GridDev).
+// 16 <= n1 <= 512, reading the tiled row layout (see GridDev) with the
+// Nyquist column packed into column 0; false when the plan does not pack.
 template <typename T>
-void launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
+bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_stride, int B,
                         int64_t n_samples, int R, int r0, const T* gM, const T* gV,
                         const double* cV, const double* vscale, const T* tw, const double* invk,
                         double* vmr, hipStream_t s);

@@ -247,6 +247,11 @@ CASES = [
     # row length has no Nyquist column
     ("3d-24x20x18", ([-4.0] * 3, [4.0] * 3, [24, 20, 18]), lambda: W.correlated_3d(20000, 11), 20, 60),
     ("2d-45x40", ([-6.0, -6.0], [6.0, 6.0], [45, 40]), lambda: W.mixture_2d(20000, 12), 30, 100),
+    # prime factors above 13 (direct-DFT passes): the reference's 101-point
+    # grids (parallelkde_tests.jl:4, directspace_tests.jl:70), 34 = 2 x 17
+    ("1d-101", ([-6.0], [6.0], [101]), lambda: W.gaussian_1d(10000, 21, lo=-6.0, hi=6.0), 40, 120),
+    ("2d-101x34", ([-6.0, -6.0], [6.0, 6.0], [101, 34]), lambda: W.mixture_2d(20000, 22), 30, 100),
+    ("3d-17x19x23", ([-4.0] * 3, [4.0] * 3, [17, 19, 23]), lambda: W.correlated_3d(20000, 23), 20, 60),
 ]
 
 
@@ -269,7 +274,7 @@ def test_estimate_matches_oracle(pk, case, precision):
         lo, hi, n = spec
         _check_fp64(m, lambda: _rounding_envelope(
             O.make_grid(lo, hi, n), x, O.bootstrap_indices(3, x.shape[0], B), steps + 1,
-            n_bootstraps=B, n_steps=steps))
+            deposits=True, n_bootstraps=B, n_steps=steps))
     else:
         # FP32 ladder FFTs: the convergence test divides second differences of
         # the log-VMR by dlogt^2 (~1e3), so FP32 rounding moves the chosen rung
@@ -339,6 +344,10 @@ VMR_CASES = [
     ("1d-256", ([-5.0], [5.0], [256]), lambda: W.gaussian_1d(5000, 4)),
     ("1d-300", ([-5.0], [5.0], [300]), lambda: W.gaussian_1d(5000, 4)),
     ("2d-150x96", ([-6.0, -6.0], [6.0, 6.0], [150, 96]), lambda: W.mixture_2d(8000, 7)),
+    # 101 points (prime) over a span where the ladder passes two cells
+    ("1d-101", ([-3.5], [3.5], [101]), lambda: W.gaussian_1d(5000, 24, lo=-3.5, hi=3.5)),
+    ("2d-101x64", ([-4.5, -4.5], [4.5, 4.5], [101, 64]),
+     lambda: W.mixture_2d(8000, 25, lo=-4.5, hi=4.5)),
 ]
 
 
@@ -380,13 +389,29 @@ def test_estimate_out_of_grid_raises(pk):
         pk.estimate_density(est, "gradepro", n_bootstraps=4, n_steps=10)
 
 
-def test_grid_with_large_prime_factor_rejected(pk):
-    """Axis lengths are limited to 13-smooth numbers (the mixed-radix FFT's
-    radices); 34 = 2 x 17 is refused with an ArgumentError."""
+def test_grid_axis_above_4096_rejected(pk):
+    """Axis lengths are limited to [2, 4096] (any factorization); 4099 is
+    refused with an ArgumentError."""
     x = np.random.default_rng(0).standard_normal((100, 1))
-    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-6.0, 6.0, 34)]))
+    est = pk.initialize_estimation(x, grid=pk.initialize_grid([(-6.0, 6.0, 4099)]))
     with pytest.raises(pk.ArgumentError):
         pk.estimate_density(est, "gradepro")
+
+
+@pytest.mark.parametrize("n_dims", [1, 2])
+def test_prime_grid_from_dims_estimates(pk, n_dims):
+    """parallelkde_tests.jl:1-13: initialize_estimation(data, dims = (101, ...),
+    grid = true) builds its grid with find_grid; the estimate on that
+    101-point (prime) grid runs and its density is finite and non-negative
+    with unit mass."""
+    x = np.random.default_rng(30 + n_dims).standard_normal((1000, n_dims))
+    est = pk.initialize_estimation(x, grid=True, dims=(101,) * n_dims)
+    assert pk.has_grid(est) and est.grid.shape == (101,) * n_dims
+    pk.estimate_density(est, "gradepro", n_bootstraps=20, n_steps=60)
+    d = pk.get_density(est)
+    assert d.shape == (101,) * n_dims and np.all(np.isfinite(d)) and np.all(d >= 0)
+    mass = d.sum() * np.prod(pk.spacings(est.grid))
+    assert abs(mass - 1.0) < 0.1
 
 
 @pytest.mark.parametrize("n_dims", [1, 2, 3])

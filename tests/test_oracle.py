@@ -51,7 +51,7 @@ def test_fft_matches_numpy_golden():
         assert np.abs(b - d["inv_" + key]).max() <= 1e-13 * np.abs(d["inv_" + key]).max(), key
 
 
-@pytest.mark.parametrize("n", [2, 3, 5, 7, 12, 16, 30, 49])
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 12, 16, 17, 30, 34, 49, 101])
 def test_fft_brute_force_dft(n):
     rng = np.random.default_rng(n)
     a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -61,7 +61,7 @@ def test_fft_brute_force_dft(n):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
-@pytest.mark.parametrize("n", [12, 30, 150, 96, 300, 64])
+@pytest.mark.parametrize("n", [12, 30, 150, 96, 300, 64, 101, 34])
 def test_fft_other_radix_orders_are_the_same_dft(n, mode):
     """The other radix orders (used to measure the rounding envelope) are the
     same transform: brute-force DFT, within FP64 rounding; they round
@@ -76,7 +76,7 @@ def test_fft_other_radix_orders_are_the_same_dft(n, mode):
     finally:
         O.set_fft_order(0)
     assert np.allclose(f, W @ a, rtol=0, atol=1e-12 * np.abs(a).sum())
-    changes = (mode == 1 and n in (30, 150, 300)) or (mode == 2 and n in (12, 96, 300, 64))
+    changes = (mode == 1 and n in (30, 150, 300, 101, 34)) or (mode == 2 and n in (12, 96, 300, 64))
     if changes:
         assert not np.array_equal(f, O.fft_nd(a, -1))
 
