@@ -761,6 +761,8 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.dst_fsize = mode == 1 ? g.fsY : (int64_t)a.H1 * g.Q;
   a.I = 1;
   a.O = g.n[2];
+  // the block index is split with shifts and masks: O must be a power of two
+  if (a.O & (a.O - 1)) return false;
   a.TI = 1;
   a.mode = mode;
   const int per_block = (kThreads * Ef) / a.L;
@@ -913,13 +915,12 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   a.Q = g.Q;
   a.H1 = g.H1;
   a.dst_fsize = a.fsize;
-  if (axis == 1) {  // grid axis 2: contiguous, outer index j3
-    a.I = 1;
-    a.O = g.n[2];
-  } else {          // grid axis 3: stride n2, inner index j2
-    a.I = g.n[1];
-    a.O = 1;
-  }
+  // q = j2 + n2 (j3 + n3 ...): inner index = the grid axes 2..axis, outer =
+  // the axes after it
+  a.I = 1;
+  a.O = 1;
+  for (int i = 1; i < axis; ++i) a.I *= g.n[i];
+  for (int i = axis + 1; i < g.N; ++i) a.O *= g.n[i];
   // pencils per workgroup: one per L/E lanes (E = 8 in FP64, 16 in FP32)
   const int Ef = sizeof(TI) == 8 ? 8 : 16;
   const int per_block = (kThreads * Ef) / a.L > 0 ? (kThreads * Ef) / a.L : 1;
@@ -928,7 +929,10 @@ void launch_axis_fft(const GridDev& g, int axis, const cplx<TI>* src, int64_t sr
   const bool gen = (a.L & (a.L - 1)) != 0;
   a.fac = g.fac[axis];
   a.twg = twg ? (const cplx<TI>*)twg + g.koff[axis] : nullptr;
-  if (!gen && a.L >= Ef && a.L <= 64 * Ef) {
+  // the wave-local form splits the block index with shifts and masks, so the
+  // other axes' extents I and O must be powers of two as well
+  const bool pow2_io = (a.I & (a.I - 1)) == 0 && (a.O & (a.O - 1)) == 0;
+  if (!gen && pow2_io && a.L >= Ef && a.L <= 64 * Ef) {
     // wave-local form: TI and TK powers of two, TK * TI <= per_block
     a.TI = a.I == 1 ? 1 : (int)std::min<int64_t>(a.I, std::min(8, per_block));
     int hp = 1;

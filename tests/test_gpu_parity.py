@@ -252,6 +252,10 @@ CASES = [
     ("1d-101", ([-6.0], [6.0], [101]), lambda: W.gaussian_1d(10000, 21, lo=-6.0, hi=6.0), 40, 120),
     ("2d-101x34", ([-6.0, -6.0], [6.0, 6.0], [101, 34]), lambda: W.mixture_2d(20000, 22), 30, 100),
     ("3d-17x19x23", ([-4.0] * 3, [4.0] * 3, [17, 19, 23]), lambda: W.correlated_3d(20000, 23), 20, 60),
+    # a power-of-two axis between axes that are not (axis passes whose other
+    # extents are not powers of two)
+    ("3d-20x32x18", ([-4.0] * 3, [4.0] * 3, [20, 32, 18]), lambda: W.correlated_3d(20000, 26), 20, 60),
+    ("3d-24x20x16", ([-4.0] * 3, [4.0] * 3, [24, 20, 16]), lambda: W.correlated_3d(20000, 27), 20, 60),
 ]
 
 
