@@ -34,8 +34,8 @@
 extern "C" {
 #endif
 
-#define PKDE_ABI_VERSION 1
-#define PKDE_MAX_DIMS 3
+#define PKDE_ABI_VERSION 2
+#define PKDE_MAX_DIMS 6
 
 typedef enum {
   PKDE_OK = 0,
@@ -110,8 +110,8 @@ int pkde_find_grid(pkde_ctx *ctx, int ndims, const double *d_samples, int64_t n,
 /* A plan binds a grid (Grid: Grids.jl:67-87 -- per-axis lo, hi, spacing and
  * point count) and the estimator keywords.  Axis sizes must lie in [2, 4096]
  * (any factorization: prime factors above 13 take direct-DFT passes; other
- * sizes return PKDE_UNSUPPORTED); ndims in 1..3 (the reference's GPU path is capped at 3-D,
- * GradeProEstimator.jl:811-814).  Replaces initialize_estimator
+ * sizes return PKDE_UNSUPPORTED); ndims in 1..PKDE_MAX_DIMS (the reference's CPU path
+ * takes any dimension; its GPU path stops at 3-D, GradeProEstimator.jl:811-814).  Replaces initialize_estimator
  * (GradeProEstimator.jl:700-735) for the :rocm device. */
 int pkde_gradepro_plan_create(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
                               const double *lo, const double *hi, const double *h,

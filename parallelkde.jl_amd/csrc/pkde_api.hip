@@ -89,8 +89,8 @@ static void factorize_wave(int L, FftFactors& f) {
 static bool rows_path(const GridDev& g, int prec) {
   const int Ew = prec ? 8 : 16;
   auto wave_ok = [&](int L) { return L >= Ew && L <= 64 * Ew; };
-  return g.N >= 2 && g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 && wave_ok(g.n[1]) &&
-         (g.N == 2 || wave_ok(g.n[2]));
+  return (g.N == 2 || g.N == 3) && g.pow2 && g.Q >= 64 && g.n[0] >= 16 && g.n[0] <= 512 &&
+         wave_ok(g.n[1]) && (g.N == 2 || wave_ok(g.n[2]));
 }
 
 static int ilog2(int64_t v) {
@@ -147,7 +147,7 @@ uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
   b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
   if (pl->g.N >= 2) b += (uint64_t)y_units(pl) * y_part_bytes(pl);
-  if (pl->fixed) b += binsort_bytes(pl->g, n, pl->B);  // sorted-binning workspace
+  if (pl->fixed && pl->g.N <= 3) b += binsort_bytes(pl->g, n, pl->B);  // sorted binning
   return b;
 }
 
@@ -204,7 +204,7 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   if ((rc = dalloc((void**)&d.params, 256, "params"))) return rc;
   if ((rc = dalloc((void**)&d.mean, 256, "mean"))) return rc;
   if ((rc = dalloc((void**)&d.partial, (size_t)kMomentBlocks * PKDE_MAX_DIMS * 8, "partial"))) return rc;
-  if (pl->fixed) {
+  if (pl->fixed && pl->g.N <= 3) {  // sorted binning: 1-3-D tiles
     const uint64_t need = binsort_bytes(pl->g, n, pl->B);
     if (need > d.binwork_bytes) {
       dfree(d.binwork);
@@ -260,8 +260,8 @@ static int enqueue_means(pkde_plan* pl, const cplx<T>* spec, const T* gM, const 
     for (int r0 = 0; r0 < S; r0 += Rm) {
       const int R = std::min(Rm, S - r0);
       launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, twg, s);
-      if (N == 3)
-        launch_axis_fft<T, T, 1>(g, 2, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, twg,
+      for (int ax = 2; ax < N; ++ax)  // the other axes in place
+        launch_axis_fft<T, T, 1>(g, ax, Y, fs, fM, Y, fs, fM, fM, 1, R, r0, gM, gM, fM + 1, tw, twg,
                                  s);
       launch_means_select<T>(g, Y, fs, R, r0, S, gM, tw, twg, d.sel, d_density, s);
     }
@@ -354,7 +354,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
           continue;
         }
       }
-      cplx<T>* Y1 = N == 2 ? Y : Y2;
+      cplx<T>* Y1 = N == 3 ? Y2 : Y;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
@@ -368,6 +368,11 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
         else if (!launch_axis3_rows<T>(g, Y2, per_rung, Y, per_rung, 2 * B, R, r0, gM, gV, B, tw,
                                        s))
           return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
+      } else {
+        // 4-D and up: the remaining axes in place
+        for (int ax = 2; ax < N; ++ax)
+          launch_axis_fft<T, T, 1>(g, ax, Y, per_rung, 0, Y, per_rung, 0, 0, 2 * B, R, r0, gM, gV,
+                                   B, tw, twg, s);
       }
       if (rows) {
         if (!launch_ladder_rows<T>(g, Y, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk,
@@ -627,7 +632,7 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
     return fail(PKDE_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   if (ndims < 1 || ndims > PKDE_MAX_DIMS)
-    return fail(PKDE_UNSUPPORTED, "the :rocm estimator supports 1 to 3 dimensions");
+    return fail(PKDE_UNSUPPORTED, "the :rocm estimator supports 1 to 6 dimensions");
   pkde_gradepro_params p;
   if (params)
     p = *params;
@@ -965,7 +970,7 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
                          double* squares_out) {
   if (!ctx || !n_per_axis || !samples || !bootstrap_idx || !sums_out || B < 1 || n < 1)
     return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
-  if (ndims < 1 || ndims > PKDE_MAX_DIMS) return fail(PKDE_UNSUPPORTED, "1 to 3 dimensions");
+  if (ndims < 1 || ndims > PKDE_MAX_DIMS) return fail(PKDE_UNSUPPORTED, "1 to 6 dimensions");
   std::lock_guard<std::mutex> lk(ctx->mu);
   PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   GridDev g{};

@@ -430,7 +430,7 @@ uint64_t binsort_bytes(const GridDev& g, int64_t n, int B) {
 int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
                           uint64_t seed, int frac_bits, void* acc, int* err, void* work,
                           uint64_t work_bytes, hipStream_t s) {
-  if (binsort_bytes(g, n, B) > work_bytes) return -1;
+  if (g.N > 3 || binsort_bytes(g, n, B) > work_bytes) return -1;
   if (g.G + 2 >= (1ll << 31) || n >= (1ll << 31) || g.n[0] > kTileCells) return -1;
   const int64_t npad = (n + 3) & ~(int64_t)3;
   unsigned char* p = (unsigned char*)work;

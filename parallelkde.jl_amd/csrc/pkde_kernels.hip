@@ -736,7 +736,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[1];
   a.log2L = g.log2n[1];
-  if (a.L < Ef || a.L > 64 * Ef || (a.L & (a.L - 1))) return false;
+  if (g.N > 3 || a.L < Ef || a.L > 64 * Ef || (a.L & (a.L - 1))) return false;
   a.src = spec;
   a.dst = dst;
   a.src_rs = 0;

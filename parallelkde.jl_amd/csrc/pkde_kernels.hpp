@@ -8,7 +8,7 @@
 
 namespace pkde {
 
-constexpr int kMaxDims = 3;
+constexpr int kMaxDims = 6;
 constexpr int kThreads = 256;
 
 // Grid geometry as the kernels see it (Grids.jl:67-87: lo, hi, spacing, n).

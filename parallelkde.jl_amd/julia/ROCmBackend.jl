@@ -112,7 +112,7 @@ struct PkdeParams
     n_bootstraps::Int32
     n_steps::Int32
     bw_step::Float64
-    bw_final::NTuple{3,Float64}
+    bw_final::NTuple{6,Float64}
     alpha_s::Float64
     alpha_os::Float64
     eps::Float64
@@ -152,14 +152,14 @@ function DensityEstimators.initialize_estimator(
     kwargs...,
 ) where {N}
     grid === nothing && throw(ArgumentError("KDE device IsROCm() does not match Grid device nothing."))
-    N <= 3 || throw(ArgumentError("ROCm implementation for dimensions higher than 3 is not supported."))
+    N <= 6 || throw(ArgumentError("ROCm implementation for dimensions higher than 6 is not supported."))
     g = grid.grid
     n = Int64[length(g.coordinates[i]) for i in 1:N]
     lo = Float64[Grids.low_bounds(g)[i] for i in 1:N]
     hi = Float64[Grids.high_bounds(g)[i] for i in 1:N]
     h = Float64[Grids.spacings(g)[i] for i in 1:N]
-    bwf = bw_final === nothing ? (NaN, NaN, NaN) :
-          ntuple(i -> Float64(bw_final isa Real ? bw_final : get(bw_final, i, NaN)), 3)
+    bwf = bw_final === nothing ? ntuple(_ -> NaN, 6) :
+          ntuple(i -> Float64(bw_final isa Real ? bw_final : get(bw_final, i, NaN)), 6)
     p = PkdeParams(
         Int32(n_bootstraps),
         Int32(n_steps === nothing ? (bw_step === nothing ? 250 : 0) : n_steps),

@@ -13,7 +13,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get(
     "PKDE_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libparallelkde_hip.so"))
-MAX_DIMS = 3
+MAX_DIMS = 6
+ABI_VERSION = 2
 
 OK = 0
 INVALID_ARGUMENT = 1
@@ -135,7 +136,7 @@ def load() -> C.CDLL:
     L.pkde_server_wait.argtypes = [vp, C.c_uint64]
     L.pkde_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
     L.pkde_server_destroy.argtypes = [vp]
-    if L.pkde_version() != 1:
+    if L.pkde_version() != ABI_VERSION:
         raise OSError("libparallelkde_hip ABI version mismatch")
     _lib = L
     return L

@@ -49,6 +49,19 @@ def correlated_3d(n, seed, lo=-4.0, hi=4.0, margin=0.07, background=0.1):
     return x[rng.permutation(n)]
 
 
+def gaussian_nd(n, d, seed, lo=-4.0, hi=4.0, margin=0.07, rho=0.3):
+    """N(0, S) in d dimensions, S = (1 - rho) I + rho 11^T; draws outside the
+    box (lo, hi - margin) on any axis are redrawn."""
+    rng = np.random.default_rng(seed)
+    Lc = np.linalg.cholesky((1.0 - rho) * np.eye(d) + rho)
+    x = rng.standard_normal((n, d)) @ Lc.T
+    bad = np.any((x <= lo) | (x >= hi - margin), axis=1)
+    while bad.any():
+        x[bad] = rng.standard_normal((bad.sum(), d)) @ Lc.T
+        bad = np.any((x <= lo) | (x >= hi - margin), axis=1)
+    return x
+
+
 def compare(dens_gpu, chosen_gpu, dens_ref, chosen_ref, S, floor=1e-3):
     """Relative L1 / Linf density error and chosen-rung agreement, overall and
     on points whose oracle density is at least `floor` of the maximum."""

@@ -256,6 +256,11 @@ CASES = [
     # extents are not powers of two)
     ("3d-20x32x18", ([-4.0] * 3, [4.0] * 3, [20, 32, 18]), lambda: W.correlated_3d(20000, 26), 20, 60),
     ("3d-24x20x16", ([-4.0] * 3, [4.0] * 3, [24, 20, 16]), lambda: W.correlated_3d(20000, 27), 20, 60),
+    # more than three dimensions (the reference's CPU path takes any N)
+    ("4d-16x16x8x8", ([-4.0] * 4, [4.0] * 4, [16, 16, 8, 8]), lambda: W.gaussian_nd(20000, 4, 28),
+     20, 60),
+    ("4d-12x10x9x8", ([-4.0] * 4, [4.0] * 4, [12, 10, 9, 8]), lambda: W.gaussian_nd(20000, 4, 29),
+     20, 60),
 ]
 
 

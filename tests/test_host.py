@@ -26,7 +26,11 @@ def test_library_exports_header_symbols():
     for name in declared:
         assert hasattr(L, name), name
     assert set(declared) == set(_lib.EXPORTED_SYMBOLS)
-    assert L.pkde_version() == 1
+    text = open(HEADER).read()
+    abi = int(re.search(r"#define PKDE_ABI_VERSION (\d+)", text).group(1))
+    dims = int(re.search(r"#define PKDE_MAX_DIMS (\d+)", text).group(1))
+    assert L.pkde_version() == abi == _lib.ABI_VERSION
+    assert dims == _lib.MAX_DIMS  # the ctypes structs' per-axis arrays
 
 
 def test_params_default_matches_reference_defaults():
