@@ -538,6 +538,10 @@ ROT_CASES = [
     ("2d-128", lambda: W.mixture_2d(20000, 4), ([(-6.0, 6.0, 128)] * 2)),
     ("3d-32", lambda: W.correlated_3d(5000, 5), ([(-4.0, 4.0, 32)] * 3)),
     ("2d-150", lambda: W.mixture_2d(20000, 10), ([(-6.0, 6.0, 150)] * 2)),
+    ("4d-16x12x10x8", lambda: W.gaussian_nd(20000, 4, 31),
+     [(-4.0, 4.0, 16), (-4.0, 4.0, 12), (-4.0, 4.0, 10), (-4.0, 4.0, 8)]),
+    ("3d-101x17x8", lambda: W.correlated_3d(5000, 32),
+     [(-4.0, 4.0, 101), (-4.0, 4.0, 17), (-4.0, 4.0, 8)]),
 ]
 
 
