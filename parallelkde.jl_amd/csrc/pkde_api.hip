@@ -115,9 +115,11 @@ static int frac_bits_for(int64_t n) {
 static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; }
 
 // The rung-chunk buffer has y_units() equal parts (two in 3-D: the axis-2
-// result and the axis-3 result in the tiled row layout); each part holds at least one
+// result and the axis-3 result in the tiled row layout; one where the fused
+// slab kernel reads the axis-2 result directly); each part holds at least one
 // rung of the 2B bootstrap fields.
-static int y_units(const pkde_plan* pl) { return pl->g.N == 3 ? 2 : 1; }
+static bool use_slab3(const pkde_plan* pl) { return pl->prec && slab3_ok(pl->g); }
+static int y_units(const pkde_plan* pl) { return pl->g.N == 3 && !use_slab3(pl) ? 2 : 1; }
 
 // Rungs per chunk: as many as the chunk budget holds, and -- unless the
 // budget was set explicitly -- at least 4 when they fit in 1/8 of the device
@@ -332,6 +334,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const bool rows = g.pack != 0;  // the power-of-two row path (rows_path)
     const bool rows3 = N == 3 && rows;
     const bool rows_gen = sizeof(T) == 8 && ladder_gen_rows_ok(g);
+    const bool slab = sizeof(T) == 8 && use_slab3(pl);  // 3-D: axes 3 and 1 fused
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
     const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
@@ -354,12 +357,20 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
           continue;
         }
       }
-      cplx<T>* Y1 = N == 3 ? Y2 : Y;
+      cplx<T>* Y1 = N == 3 && !slab ? Y2 : Y;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
                                  twg, s);
+      }
+      if constexpr (sizeof(T) == 8) {
+        if (slab) {
+          if (!launch_slab3(g, Y1, per_rung, B, n, R, r0, gM, gV, d.cV, d.vscale, tw, d.invk, d.vmr,
+                            s))
+            return fail(PKDE_DEVICE_ERROR, "slab kernel layout mismatch");
+          continue;
+        }
       }
       if (N == 3) {
         if (!rows3)

@@ -1053,6 +1053,7 @@ struct VmrArgs {
   // rows3: rows are q' = j3 + n3 * j2 (3-D), grid row q = j2 + n2 * j3
   int log2TKy, rows3, log2n2, log2n3;
   int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
+  int goff3;     // offset of grid axis 3 in the per-rung Gaussian tables
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -1704,6 +1705,201 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
     rows_dispatch<T, E, 4>(g.log2n[0], blocks, lds, s, a);
   else
     rows_dispatch<T, E, 8>(g.log2n[0], blocks, lds, s, a);
+  return true;
+}
+
+// ------------------------------------------------ 3-D ladder, small slabs
+// The 3-D ladder step after the axis-2 pass for grids whose (k1, k3) slab of
+// a field pair fits LDS (n1 n3 up to 64 x 64, 128 x 32 or 256 x 16 in FP64):
+// one workgroup of 16 waves per (rung, j2) walks the bootstraps in order.
+// Per bootstrap b: the (k1, k3) planes of column j2 of its sums and squares
+// fields ([k1][j2][k3] from the axis-2 pass, k3 contiguous) go to LDS with
+// their axis-3 Gaussian factors, their axis-3 inverse transform runs
+// wave-locally on the 2 n1/2 pencils, then each wave builds the axis-1 rows
+// of its n3/16 values of j3 with the sums field in the real part and the
+// squares field in the imaginary part (each with its axis-1 factor; Nyquist
+// column packed into column 0, as k_ladder_rows), transforms them, and
+// forms the VMR ratio and Welford update of each point.  Against the axis-3
+// pass into tiled rows followed by k_ladder_rows, the rung's fields are read
+// once and not written again.
+constexpr int kSlabThreads = 1024;
+
+template <int LOG2N1, int LOG2N3>
+__global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
+  constexpr int n1 = 1 << LOG2N1, H = n1 / 2, n3 = 1 << LOG2N3;
+  constexpr int E = 8;
+  constexpr int PL1 = pencil_stride(n1), PL3 = pencil_stride(n3);
+  constexpr int NW = kSlabThreads / 64;
+  constexpr int RW = n3 / NW;            // rows (values of j3) per wave
+  constexpr int PPT = RW * n1 / 64;      // points per lane
+  constexpr int lp1 = n1 / E, ppc1 = 64 / lp1;
+  constexpr int lp3 = n3 / E, ppc3 = 64 / lp3;
+  static_assert(RW >= 1 && PPT >= 1 && RW * n1 == 64 * PPT, "slab shape");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<double>* P = (cplx<double>*)smem;        // [2][H][PL3]: the pair's planes
+  cplx<double>* Zall = P + 2 * H * PL3;          // [NW][RW][PL1]: axis-1 rows
+  cplx<double>* twl1 = Zall + NW * RW * PL1;
+  cplx<double>* twl3 = twl1 + n1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int k = tid; k < n1; k += kSlabThreads) twl1[k] = a.tw[n1 - 2 + k];
+  for (int k = tid; k < n3; k += kSlabThreads) twl3[k] = a.tw[n3 - 2 + k];
+  const int n2 = a.n2r;
+  const int rr = (int)blockIdx.x / n2, j2 = (int)blockIdx.x - rr * n2;
+  const int r = a.r0 + rr;
+  const double* gm = a.gM + (int64_t)r * a.ksum;
+  const double* gv = a.gV + (int64_t)r * a.ksum;
+  const double vfac = a.cV[r] * a.invG;
+  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * n3;
+  cplx<double>* Z = Zall + wv * RW * PL1;
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+  const int B = a.B;
+  for (int b = 0; b < B; ++b) {
+    const double inv = a.invk[b];
+    __syncthreads();  // every wave is done with the previous planes
+    // plane 0: sums field b (means factors); plane 1: squares field B + b
+    for (int id = tid; id < 2 * H * n3; id += kSlabThreads) {
+      const int p = id >> (LOG2N1 - 1 + LOG2N3);
+      const int k1 = (id >> LOG2N3) & (H - 1);
+      const int k3 = id & (n3 - 1);
+      const double g3 = (p ? gv : gm)[a.goff3 + k3];
+      P[(p * H + k1) * PL3 + pad8(k3)] =
+          cscale(src[(int64_t)(b + p * B) * a.fsize + (int64_t)k1 * a.Q + k3], g3);
+    }
+    __syncthreads();
+    for (int c = wv * ppc3; c < 2 * H; c += NW * ppc3) {
+      const int pen = c + lane / lp3;
+      const bool act = pen < 2 * H;
+      wave_fft<LOG2N3, 1, double, E>(P + (act ? pen : 0) * PL3, lane % lp3, act, twl3);
+    }
+    __syncthreads();
+    for (int id = lane; id < RW * H; id += 64) {
+      const int w = id >> (LOG2N1 - 1), k1 = id & (H - 1);
+      const int j3 = wv * RW + w;
+      const cplx<double> pa = P[k1 * PL3 + pad8(j3)];
+      const cplx<double> pb = P[(H + k1) * PL3 + pad8(j3)];
+      cplx<double>* row = Z + w * PL1;
+      if (k1 == 0) {
+        // packed column: real part k1 = 0, imaginary part k1 = n1/2
+        const double g0m = gm[0], gNm = gm[H], g0v = gv[0], gNv = gv[H];
+        row[pad8(0)] = {pa.re * g0m, pb.re * g0v};
+        row[pad8(H)] = {pa.im * gNm, pb.im * gNv};
+      } else {
+        const cplx<double> A = cscale(pa, gm[k1]), Bv = cscale(pb, gv[k1]);
+        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < RW; c += ppc1) {
+      const int pen = c + lane / lp1;
+      const bool act = pen < RW;
+      wave_fft<LOG2N1, 1, double, E>(Z + (act ? pen : 0) * PL1, lane % lp1, act, twl1);
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int pt = lane + 64 * u;
+      const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
+      const cplx<double> z = Z[w * PL1 + pad8(j1)];  // sums + i squares
+      const double xv = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
+      const double d = xv - mean[u];
+      mean[u] += d * inv;
+      m2[u] += d * (xv - mean[u]);
+    }
+    wave_sync();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int pt = lane + 64 * u;
+    const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
+    const int64_t q = (int64_t)j2 + (int64_t)n2 * (wv * RW + w);  // grid row j2 + n2 j3
+    const double vv = sc * m2[u] / (double)(B - 1);
+    a.vmr[(int64_t)r * a.G + q * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+  }
+}
+
+static size_t slab3_lds(int n1, int n3) {
+  const int H = n1 / 2, RW = n3 / (kSlabThreads / 64);
+  return ((size_t)2 * H * pencil_stride(n3) + (size_t)(kSlabThreads / 64) * RW * pencil_stride(n1) +
+          n1 + n3) *
+         sizeof(cplx<double>);
+}
+
+// Shapes (log2 n1, log2 n3) with an instantiation of k_slab3.
+static bool slab3_shape(int l1, int l3) {
+  switch (l1 * 16 + l3) {
+    case 4 * 16 + 6: case 5 * 16 + 5: case 5 * 16 + 6: case 6 * 16 + 4: case 6 * 16 + 5:
+    case 6 * 16 + 6: case 7 * 16 + 4: case 7 * 16 + 5: case 8 * 16 + 4:
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool slab3_ok(const GridDev& g) {
+  return g.N == 3 && g.pack && g.pow2 && slab3_shape(g.log2n[0], g.log2n[2]) &&
+         slab3_lds(g.n[0], g.n[2]) <= 160 * 1024;
+}
+
+template <int LOG2N1, int LOG2N3>
+static void slab3_launch(unsigned blocks, hipStream_t s, const VmrArgs<double>& a) {
+  constexpr int n1 = 1 << LOG2N1, n3 = 1 << LOG2N3;
+  static const bool attr = [] {
+    // more than 64 KB of LDS per workgroup; set once per instantiation
+    (void)hipFuncSetAttribute((const void*)k_slab3<LOG2N1, LOG2N3>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)slab3_lds(n1, n3));
+    return true;
+  }();
+  (void)attr;
+  k_slab3<LOG2N1, LOG2N3><<<blocks, kSlabThreads, slab3_lds(n1, n3), s>>>(a);
+}
+
+bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride, int B,
+                  int64_t n_samples, int R, int r0, const double* gM, const double* gV,
+                  const double* cV, const double* vscale, const double* tw, const double* invk,
+                  double* vmr, hipStream_t s) {
+  if (!slab3_ok(g)) return false;
+  VmrArgs<double> a{};
+  a.src = src;
+  a.src_rs = src_rung_stride;
+  a.fsize = (int64_t)g.Hc * g.Q;  // the axis-2 pass's output: n1/2 columns per field
+  a.B = B;
+  a.invn = 1.0 / (double)n_samples;
+  a.invG = 1.0 / (double)g.G;
+  a.R = R;
+  a.r0 = r0;
+  a.gM = gM;
+  a.gV = gV;
+  a.ksum = g.ksum;
+  a.goff3 = g.koff[2];
+  a.cV = cV;
+  a.vscale = vscale;
+  a.tw = (const cplx<double>*)tw;
+  a.invk = invk;
+  a.vmr = vmr;
+  a.G = g.G;
+  a.Q = g.Q;
+  a.n1 = g.n[0];
+  a.n2r = g.n[1];
+  const unsigned blocks = (unsigned)((int64_t)R * g.n[1]);
+  switch (g.log2n[0] * 16 + g.log2n[2]) {
+    case 4 * 16 + 6: slab3_launch<4, 6>(blocks, s, a); break;
+    case 5 * 16 + 5: slab3_launch<5, 5>(blocks, s, a); break;
+    case 5 * 16 + 6: slab3_launch<5, 6>(blocks, s, a); break;
+    case 6 * 16 + 4: slab3_launch<6, 4>(blocks, s, a); break;
+    case 6 * 16 + 5: slab3_launch<6, 5>(blocks, s, a); break;
+    case 6 * 16 + 6: slab3_launch<6, 6>(blocks, s, a); break;
+    case 7 * 16 + 4: slab3_launch<7, 4>(blocks, s, a); break;
+    case 7 * 16 + 5: slab3_launch<7, 5>(blocks, s, a); break;
+    case 8 * 16 + 4: slab3_launch<8, 4>(blocks, s, a); break;
+    default: return false;
+  }
   return true;
 }
 

@@ -132,6 +132,16 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
                         const double* cV, const double* vscale, const T* tw, const double* invk,
                         double* vmr, hipStream_t s);
 
+// 3-D ladder step fused after the axis-2 pass (mode 2 output, [k1][j2][k3]
+// with the Nyquist column packed) for grids whose (k1, k3) slab of a field
+// pair fits LDS: axis-3 and axis-1 inverse passes, VMR and Welford in one
+// kernel (FP64).  slab3_ok tells whether the grid takes it.
+bool slab3_ok(const GridDev& g);
+bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride, int B,
+                  int64_t n_samples, int R, int r0, const double* gM, const double* gV,
+                  const double* cV, const double* vscale, const double* tw, const double* invk,
+                  double* vmr, hipStream_t s);
+
 // :rot propagation time per axis from the sample standard deviations
 // (rule 0 Scott, 1 Silverman); and the per-axis Gaussian factor tables alone.
 void launch_rot_times(const GridDev& g, int64_t n, int rule, const double* sigma, double* times,

@@ -256,6 +256,10 @@ CASES = [
     # extents are not powers of two)
     ("3d-20x32x18", ([-4.0] * 3, [4.0] * 3, [20, 32, 18]), lambda: W.correlated_3d(20000, 26), 20, 60),
     ("3d-24x20x16", ([-4.0] * 3, [4.0] * 3, [24, 20, 16]), lambda: W.correlated_3d(20000, 27), 20, 60),
+    # 3-D slabs fused after the axis-2 pass (k_slab3; 3d-32 takes it as well)
+    ("3d-64x16x64", ([-4.0] * 3, [4.0] * 3, [64, 16, 64]), lambda: W.correlated_3d(20000, 33), 20, 60),
+    ("3d-128x16x32", ([-4.0] * 3, [4.0] * 3, [128, 16, 32]), lambda: W.correlated_3d(20000, 34), 20,
+     60),
     # more than three dimensions (the reference's CPU path takes any N)
     ("4d-16x16x8x8", ([-4.0] * 4, [4.0] * 4, [16, 16, 8, 8]), lambda: W.gaussian_nd(20000, 4, 28),
      20, 60),
@@ -357,6 +361,8 @@ VMR_CASES = [
     ("1d-101", ([-3.5], [3.5], [101]), lambda: W.gaussian_1d(5000, 24, lo=-3.5, hi=3.5)),
     ("2d-101x64", ([-4.5, -4.5], [4.5, 4.5], [101, 64]),
      lambda: W.mixture_2d(8000, 25, lo=-4.5, hi=4.5)),
+    # 64^3 (the fused slab kernel) over a box where the ladder passes two cells
+    ("3d-64", ([-3.0] * 3, [3.0] * 3, [64, 64, 64]), lambda: W.correlated_3d(8000, 35, lo=-3.0, hi=3.0)),
 ]
 
 
