@@ -1758,18 +1758,39 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     m2[u] = 0.0;
   }
   const int B = a.B;
+  // plane values per thread; rows up to 64 points also hold the next
+  // bootstrap's planes in registers while this one is transformed (the
+  // longer rows would spill)
+  constexpr int LPT = 2 * H * n3 / kSlabThreads;
+  constexpr bool kPrefetch = LOG2N1 <= 6;
+  cplx<double> nx[LPT];
+  auto plane_elem = [&](int i, int& p, int& k1, int& k3) {
+    const int id = tid + i * kSlabThreads;
+    p = id >> (LOG2N1 - 1 + LOG2N3);
+    k1 = (id >> LOG2N3) & (H - 1);
+    k3 = id & (n3 - 1);
+  };
+  // plane 0: sums field bb (means factors); plane 1: squares field B + bb
+  auto load_planes = [&](int bb) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      int p, k1, k3;
+      plane_elem(i, p, k1, k3);
+      nx[i] = src[(int64_t)(bb + p * B) * a.fsize + (int64_t)k1 * a.Q + k3];
+    }
+  };
+  if constexpr (kPrefetch) load_planes(0);
   for (int b = 0; b < B; ++b) {
     const double inv = a.invk[b];
+    if constexpr (!kPrefetch) load_planes(b);
     __syncthreads();  // every wave is done with the previous planes
-    // plane 0: sums field b (means factors); plane 1: squares field B + b
-    for (int id = tid; id < 2 * H * n3; id += kSlabThreads) {
-      const int p = id >> (LOG2N1 - 1 + LOG2N3);
-      const int k1 = (id >> LOG2N3) & (H - 1);
-      const int k3 = id & (n3 - 1);
-      const double g3 = (p ? gv : gm)[a.goff3 + k3];
-      P[(p * H + k1) * PL3 + pad8(k3)] =
-          cscale(src[(int64_t)(b + p * B) * a.fsize + (int64_t)k1 * a.Q + k3], g3);
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      int p, k1, k3;
+      plane_elem(i, p, k1, k3);
+      P[(p * H + k1) * PL3 + pad8(k3)] = cscale(nx[i], (p ? gv : gm)[a.goff3 + k3]);
     }
+    if constexpr (kPrefetch) load_planes(b + 1 < B ? b + 1 : b);  // in flight meanwhile
     __syncthreads();
     for (int c = wv * ppc3; c < 2 * H; c += NW * ppc3) {
       const int pen = c + lane / lp3;
