@@ -384,8 +384,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the :gradepro path has no multi-GPU form in this "
-              "build (estimates shard by request; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: the :gradepro path has no multi-GPU form here "
+              "(independent estimates shard by request, one process per GPU).", file=sys.stderr)
         sys.exit(2)
     if args.workload == "copy":
         # calibration for the HBM byte counters: a device copy of known size
