@@ -265,6 +265,9 @@ CASES = [
      20, 60),
     ("4d-12x10x9x8", ([-4.0] * 4, [4.0] * 4, [12, 10, 9, 8]), lambda: W.gaussian_nd(20000, 4, 29),
      20, 60),
+    ("5d-8x6x5x4x4", ([-4.0] * 5, [4.0] * 5, [8, 6, 5, 4, 4]), lambda: W.gaussian_nd(5000, 5, 36),
+     20, 60),
+    ("6d-4^6", ([-4.0] * 6, [4.0] * 6, [4] * 6), lambda: W.gaussian_nd(5000, 6, 37), 20, 60),
 ]
 
 
@@ -293,9 +296,10 @@ def test_estimate_matches_oracle(pk, case, precision):
         # the log-VMR by dlogt^2 (~1e3), so FP32 rounding moves the chosen rung
         # at a tenth to a fifth of the points; this mode is outside the parity
         # bar and these bounds only document it (measured: rel L1 2e-4..5e-3,
-        # agreement 0.79-0.96 on significant points).
+        # agreement 0.72-0.96 on significant points up to 4-D; on the coarse
+        # 5-D and 6-D grids, with 4-point axes, 0.52-0.55).
         assert m["rel_l1"] <= 1e-2
-        assert m["agree_sig"] >= 0.7
+        assert m["agree_sig"] >= (0.7 if len(spec[2]) <= 4 else 0.45)
 
 
 # Estimator options the reference exposes (GradeProEstimator.jl:700-735,
