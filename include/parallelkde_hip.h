@@ -155,6 +155,16 @@ int pkde_gradepro_estimate_device(pkde_plan *plan, const double *d_samples, int6
 int pkde_gradepro_check(pkde_plan *plan);
 /* Timing and ladder scalars of the plan's last estimate (waits for it). */
 int pkde_gradepro_last_stats(pkde_plan *plan, pkde_stats *stats);
+/* Graph replay of repeated estimates (off by default): when an estimate
+ * repeats the previous call's samples, bootstrap indices and output buffers
+ * (the same device pointers and n), the library captures the estimate as
+ * three HIP graphs and later such calls replay them instead of launching
+ * each kernel; results are identical.  0 turns it off and drops the graphs.
+ * Off by default because replay was measured slower than the direct
+ * launches on this path (1-D 512: 0.288 against 0.268 ms per estimate).  No
+ * counterpart in the reference (its CUDA path launches every kernel of every
+ * rung, GradeProEstimator.jl:968-1051). */
+int pkde_gradepro_graph_enable(pkde_plan *plan, int enable);
 
 /* Per-rung log-VMR table of the last estimate (S x G, rung-major), for
  * primitive-level parity (calculate_scaled_vmr!, DirectSpace.jl:526-584).

@@ -228,7 +228,10 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   return PKDE_OK;
 }
 
+static void graphs_clear(PlanGraphs& G);
+
 static void plan_free(pkde_plan* pl) {
+  graphs_clear(pl->graphs);
   PlanBuffers& d = pl->d;
   void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.conv, d.chosen, d.sel, d.density, d.err,
                 d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
@@ -273,59 +276,62 @@ static int enqueue_means(pkde_plan* pl, const cplx<T>* spec, const T* gM, const 
 
 template <typename T>
 static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
-                         double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
+                         double* d_density, int16_t* d_chosen, hipStream_t s, bool timed,
+                         int seg = -1) {
   PlanBuffers& d = pl->d;
   const GridDev& g = pl->g;
   const int B = pl->B, S = pl->S, N = g.N;
   const int64_t G = g.G, F = 2 * (int64_t)B + 1, fs = fsize(pl);
   const T* tw = (const T*)(sizeof(T) == 8 ? (const void*)pl->ctx->tw64 : (const void*)pl->ctx->tw32);
   const T* twg = (const T*)(sizeof(T) == 8 ? (const void*)d.twg64 : (const void*)d.twg32);
+  // seg < 0: the whole estimate; 0, 1, 2: only binning and forward
+  // transforms, only the ladder, only convergence and means (graph segments)
+  const bool all = seg < 0;
 
-  if (timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
-  launch_fill_i32(d.err, 0, 1, s);
-  // the sorted fixed-point binning writes every accumulator cell itself
-  const bool sorted = pl->fixed && d.binwork;
-  if (!sorted) launch_zero_bytes(d.acc, F * G * 8, s);
-  if (!pl->bw_given) {
-    const double pref = pow(4.0 / ((double)(N + 2) * (double)n), 1.0 / (double)(N + 4));
-    launch_silverman(N, d_x, n, pref, d.partial, d.mean, d.bw, s);
-  }
-  launch_tables<T>(g, S, pl->steps_mode, pl->steps_or_step, d.bw, d.kfreq, n, pl->alpha_s,
-                   pl->alpha_os, pl->eps, d.times, d.dlogt, d.vscale, d.cV, d.params, (T*)d.gM,
-                   (T*)d.gV, s);
-  if (sorted) {
-    if (launch_binning_sorted(g, d_x, n, d_idx, B, pl->p.rng_seed, frac_bits_for(n), d.acc, d.err,
-                              d.binwork, d.binwork_bytes, s))
-      return fail(PKDE_DEVICE_ERROR, "binning workspace does not fit the estimate");
-  } else {
-    launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err,
-                   s);
-  }
-  {
-    double hp = g.h[0];
-    for (int i = 1; i < N; ++i) hp *= g.h[i];
-    const double two_fb = ldexp(1.0, frac_bits_for(n));
-    const double scaleM = (double)n * hp * two_fb;
-    const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
-    launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, B, d.specD, pl->ctx->tw64, d.twg64,
-                         s);
-  }
-  for (int ax = 1; ax < N; ++ax)
-    launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
-                                        nullptr, nullptr, 0, pl->ctx->tw64, d.twg64, s);
-  const cplx<T>* spec;
-  if (sizeof(T) == 4) {
-    launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
-    spec = (const cplx<T>*)d.spec;
-  } else {
-    spec = (const cplx<T>*)d.specD;
-  }
-  if (timed) PKDE_TRY(hipEventRecord(pl->ev[1], s), "event");
+  if (all && timed) PKDE_TRY(hipEventRecord(pl->ev[0], s), "event");
+  if (all || seg == 0) {
+    launch_fill_i32(d.err, 0, 1, s);
+    // the sorted fixed-point binning writes every accumulator cell itself
+    const bool sorted = pl->fixed && d.binwork;
+    if (!sorted) launch_zero_bytes(d.acc, F * G * 8, s);
+    if (!pl->bw_given) {
+      const double pref = pow(4.0 / ((double)(N + 2) * (double)n), 1.0 / (double)(N + 4));
+      launch_silverman(N, d_x, n, pref, d.partial, d.mean, d.bw, s);
+    }
+    launch_tables<T>(g, S, pl->steps_mode, pl->steps_or_step, d.bw, d.kfreq, n, pl->alpha_s,
+                     pl->alpha_os, pl->eps, d.times, d.dlogt, d.vscale, d.cV, d.params, (T*)d.gM,
+                     (T*)d.gV, s);
+    if (sorted) {
+      if (launch_binning_sorted(g, d_x, n, d_idx, B, pl->p.rng_seed, frac_bits_for(n), d.acc, d.err,
+                                d.binwork, d.binwork_bytes, s))
+        return fail(PKDE_DEVICE_ERROR, "binning workspace does not fit the estimate");
+    } else {
+      launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err,
+                     s);
+    }
+    {
+      double hp = g.h[0];
+      for (int i = 1; i < N; ++i) hp *= g.h[i];
+      const double two_fb = ldexp(1.0, frac_bits_for(n));
+      const double scaleM = (double)n * hp * two_fb;
+      const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
+      launch_forward_axis1(g, d.acc, pl->fixed, scaleM, scaleV, B, d.specD, pl->ctx->tw64, d.twg64,
+                           s);
+    }
+    for (int ax = 1; ax < N; ++ax)
+      launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
+                                          nullptr, nullptr, 0, pl->ctx->tw64, d.twg64, s);
+    if (sizeof(T) == 4) launch_convert<double, float>(d.specD, (cplx<float>*)d.spec, F * fs, s);
+  }  // segment 0
+  const cplx<T>* spec = sizeof(T) == 4 ? (const cplx<T>*)d.spec : (const cplx<T>*)d.specD;
+  if (all && timed) PKDE_TRY(hipEventRecord(pl->ev[1], s), "event");
 
   // ---- ladder: log-VMR for every rung ----
   const T* gM = (const T*)d.gM;
   const T* gV = (const T*)d.gV;
-  if (N == 1) {
+  if (!all && seg != 1) {
+    // not this segment
+  } else if (N == 1) {
     launch_ladder_vmr<T>(g, spec, 0, B, n, S, 0, gM, gV, d.cV, d.vscale, tw, twg, d.vmr, s);
   } else {
     // axis 2 (and 3) inverse passes write the tiled row layout the row kernel
@@ -394,14 +400,16 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
                              s);
     }
   }
-  if (timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
+  if (all && timed) PKDE_TRY(hipEventRecord(pl->ev[2], s), "event");
 
   // ---- convergence + means at the chosen rung ----
-  int16_t* chosen = d_chosen ? d_chosen : d.chosen;
-  launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, d.conv, s);
-  const int rc = enqueue_means<T>(pl, spec, gM, tw, d_density, s);
-  if (rc) return rc;
-  if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
+  if (all || seg == 2) {
+    int16_t* chosen = d_chosen ? d_chosen : d.chosen;
+    launch_convergence(G, S, d.vmr, d.dlogt, d.params, chosen, d.sel, d.conv, s);
+    const int rc = enqueue_means<T>(pl, spec, gM, tw, d_density, s);
+    if (rc) return rc;
+  }
+  if (all && timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
   return check_hip(hipGetLastError(), "kernel launch");
 }
 
@@ -485,6 +493,85 @@ static int ladder_from_data(pkde_plan* pl, const double* d_x, int64_t n, hipStre
   return plan_reserve(pl, n, false);
 }
 
+// Signature of the plan's device buffers: captured graphs hold their
+// addresses, so a reallocation must invalidate them.
+static uint64_t buffer_sig(const pkde_plan* pl) {
+  const PlanBuffers& d = pl->d;
+  const void* ps[] = {d.x,      d.idx,    d.acc,   d.specD,  d.spec,   d.Y,      d.vmr,
+                      d.conv,   d.chosen, d.sel,   d.density, d.err,   d.bw,     d.mean,
+                      d.partial, d.kfreq, d.times, d.dlogt,  d.vscale, d.cV,     d.params,
+                      d.gM,     d.gV,     d.invk,  d.binwork, d.twg64, d.twg32};
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the addresses and sizes
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for (const void* p : ps) mix((uint64_t)(uintptr_t)p);
+  mix(d.binwork_bytes);
+  mix((uint64_t)d.Y_elems);
+  mix((uint64_t)pl->S);
+  return h;
+}
+
+static void graphs_clear(PlanGraphs& G) {
+  for (auto& e : G.exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  G.key = GraphKey();
+}
+
+// The estimate as three captured graphs, replayed while the caller repeats
+// the same buffers.  The first call with a new key runs directly; the second
+// captures; capture failures leave the plan on direct launches.  The timing
+// events are recorded between the graph launches, as in the direct form.
+template <typename T>
+static int enqueue_graphs(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
+                          double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
+  PlanGraphs& G = pl->graphs;
+  GraphKey k;
+  k.x = d_x;
+  k.n = n;
+  k.idx = d_idx;
+  k.dens = d_density;
+  k.chosen = d_chosen;
+  k.sig = buffer_sig(pl);
+  k.timed = timed;
+  if (!(G.exec[0] && k == G.key)) {
+    const bool repeat = k == G.last;
+    G.last = k;
+    if (!repeat) return enqueue_typed<T>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
+    graphs_clear(G);
+    for (int seg = 0; seg < 3; ++seg) {
+      hipGraph_t graph = nullptr;
+      int rc = PKDE_OK;
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        rc = enqueue_typed<T>(pl, d_x, n, d_idx, d_density, d_chosen, s, false, seg);
+        e = hipStreamEndCapture(s, &graph);
+      }
+      if (e == hipSuccess && rc == PKDE_OK && graph)
+        e = hipGraphInstantiate(&G.exec[seg], graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc != PKDE_OK || e != hipSuccess || !G.exec[seg]) {
+        (void)hipGetLastError();
+        graphs_clear(G);
+        G.failed = true;
+        if (rc != PKDE_OK) return rc;
+        return enqueue_typed<T>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
+      }
+    }
+    G.key = k;
+  }
+  for (int seg = 0; seg < 3; ++seg) {
+    if (timed) PKDE_TRY(hipEventRecord(pl->ev[seg], s), "event");
+    PKDE_TRY(hipGraphLaunch(G.exec[seg], s), "graph launch");
+  }
+  if (timed) PKDE_TRY(hipEventRecord(pl->ev[3], s), "event");
+  return PKDE_OK;
+}
+
 int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
                  double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
   if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
@@ -496,6 +583,9 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
   if (pl->kind == 1)
     return pl->prec ? enqueue_rot<double>(pl, d_x, n, d_density, s, timed)
                     : enqueue_rot<float>(pl, d_x, n, d_density, s, timed);
+  if (pl->graphs.enabled && !pl->graphs.failed && !pl->S_from_data && !pl->debug)
+    return pl->prec ? enqueue_graphs<double>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed)
+                    : enqueue_graphs<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
   if (pl->prec) return enqueue_typed<double>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
   return enqueue_typed<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
 }
@@ -927,6 +1017,16 @@ int pkde_gradepro_last_stats(pkde_plan* pl, pkde_stats* stats) {
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
   PKDE_TRY(hipEventSynchronize(pl->ev[3]), "estimate");
   return finish_stats(pl, pl->last_n, stats);
+}
+
+int pkde_gradepro_graph_enable(pkde_plan* pl, int enable) {
+  if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  if (pl->ctx) PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  pkde::graphs_clear(pl->graphs);
+  pl->graphs.enabled = enable != 0;
+  pl->graphs.failed = false;
+  pl->graphs.last = pkde::GraphKey();
+  return PKDE_OK;
 }
 
 int pkde_gradepro_debug_enable(pkde_plan* pl, int enable) {

@@ -82,6 +82,33 @@ struct PlanBuffers {
   int S_alloc = 0;
 };
 
+// With graphs enabled (pkde_gradepro_graph_enable), repeated estimates with
+// the same device buffers replay three captured graphs (binning and forward
+// transforms; the ladder; convergence and means) instead of launching each
+// kernel again.  A call captures when it repeats
+// the previous call's key; `sig` changes when any plan buffer is
+// reallocated, so replayed kernels never see a stale pointer.
+struct GraphKey {
+  const void* x = nullptr;
+  int64_t n = 0;
+  const void* idx = nullptr;
+  const void* dens = nullptr;
+  const void* chosen = nullptr;
+  uint64_t sig = 0;
+  bool timed = false;
+  bool operator==(const GraphKey& o) const {
+    return x == o.x && n == o.n && idx == o.idx && dens == o.dens && chosen == o.chosen &&
+           sig == o.sig && timed == o.timed;
+  }
+};
+
+struct PlanGraphs {
+  bool enabled = false;  // opt-in: direct launches were measured faster
+  bool failed = false;  // a capture failed: this plan stays on direct launches
+  GraphKey last, key;
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+};
+
 }  // namespace pkde
 
 struct pkde_plan {
@@ -107,6 +134,7 @@ struct pkde_plan {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t last_n = 0;
   int last_rc = PKDE_OK;
+  pkde::PlanGraphs graphs;
 };
 
 namespace pkde {

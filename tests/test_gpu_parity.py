@@ -470,6 +470,60 @@ def test_find_grid_on_device_matches_host(pk, L, n_dims):
         hip.hipFree(d)
 
 
+@pytest.mark.parametrize("shape", [[(-6.0, 6.0, 64)] * 2, [(-4.0, 4.0, 32)] * 3,
+                                   [(-5.0, 5.0, 512)]], ids=["2d-64", "3d-32", "1d-512"])
+def test_graph_replay_matches_direct(pk, L, shape):
+    """Repeated estimates on the same device buffers replay captured graphs
+    (pkde_gradepro_graph_enable): every replay, with another plan's estimate
+    and a synchronisation between replays, equals the direct launches bit for
+    bit, and the ladder timing is still recorded."""
+    from parallelkde_rocm import _lib
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library itself uses
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nd = len(shape)
+    x = np.ascontiguousarray({1: lambda: W.gaussian_1d(10000, 42)[:, None],
+                              2: lambda: W.mixture_2d(20000, 43),
+                              3: lambda: W.correlated_3d(20000, 44)}[nd]().reshape(-1, nd))
+    grid = pk.initialize_grid(shape)
+    G = int(np.prod(grid.shape))
+    plan = pk._Plan(grid, pk._params(nd, n_bootstraps=20, n_steps=60))
+    other = pk._Plan(pk.initialize_grid([(-5.0, 5.0, 128)]), pk._params(1, n_bootstraps=10, n_steps=40))
+    xo = np.ascontiguousarray(W.gaussian_1d(3000, 45)[:, None])
+    ptrs = [C.c_void_p() for _ in range(3)]
+    sizes = [x.nbytes, G * 8, G * 2]
+    for p, nb in zip(ptrs, sizes):
+        assert hip.hipMalloc(C.byref(p), nb) == 0
+    d_x, d_dens, d_ch = ptrs
+    try:
+        assert hip.hipMemcpy(d_x, x.ctypes.data, x.nbytes, 1) == 0
+
+        def run():
+            assert L.pkde_gradepro_estimate_device(plan.handle, d_x, x.shape[0], None, d_dens, d_ch,
+                                                   None) == 0
+            st = _lib.Stats()
+            assert L.pkde_gradepro_last_stats(plan.handle, C.byref(st)) == 0
+            dens = np.empty(G)
+            ch = np.empty(G, dtype=np.int16)
+            assert hip.hipMemcpy(dens.ctypes.data, d_dens, G * 8, 2) == 0  # device to host
+            assert hip.hipMemcpy(ch.ctypes.data, d_ch, G * 2, 2) == 0
+            return dens, ch, st
+
+        assert L.pkde_gradepro_graph_enable(plan.handle, 0) == 0
+        d0, c0, _ = run()
+        assert L.pkde_gradepro_graph_enable(plan.handle, 1) == 0
+        for i in range(5):  # direct, capture + replay, then replays
+            d, c, st = run()
+            assert np.array_equal(d, d0) and np.array_equal(c, c0), i
+            assert st.ms_ladder > 0 and st.ms_total >= st.ms_ladder
+            other.estimate(xo)  # other work between replays
+        say(f"  graph replays equal the direct launches ({shape})")
+    finally:
+        for p in ptrs:
+            hip.hipFree(p)
+
+
 def test_deterministic_repeat(pk):
     """Fixed-point binning + fixed reduction orders: bitwise repeatable."""
     x = W.mixture_2d(30000, 9)
