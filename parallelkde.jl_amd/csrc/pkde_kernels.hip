@@ -1191,20 +1191,44 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
     m2[u] = 0.0;
   }
   const cplx<T> zero = {(T)0, (T)0};
+  // The wave's loads of a round (sums and squares fields of its pair, the
+  // half spectrum) go to registers all at once, one round ahead: they are in
+  // flight during the previous round's Welford updates.  A wave without a
+  // pair in the round reloads bootstrap 0 (valid addresses, values unused).
+  constexpr int NIT = (2 * (H + 1) + 63) / 64;
+  cplx<T> la[NIT], lb[NIT];
+  auto issue = [&](int b) {
+    const int bb = b < B ? b : 0;
+    const int o2 = bb + 1 < B ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int id = lane + 64 * i;
+      const int idc = id < 2 * (H + 1) ? id : 0;
+      const int ty = idc > H;
+      const int k1 = ty ? idc - H - 1 : idc;
+      const int fa = ty * B + bb;
+      la[i] = src[(int64_t)fa * a.fsize + k1];
+      lb[i] = src[(int64_t)(fa + o2) * a.fsize + k1];
+    }
+  };
+  issue(2 * wv);
   for (int b0 = 0; b0 < B; b0 += 2 * kWaves1d) {
     const int b = b0 + 2 * wv;
     if (b < B) {
       const bool has2 = b + 1 < B;
-      for (int id = lane; id < 2 * (H + 1); id += 64) {
-        const int ty = id > H;
-        const int k1 = ty ? id - H - 1 : id;
-        const int fa = ty * B + b;
-        const T gk = ty ? gv1[k1] : gm1[k1];
-        const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + k1], gk);
-        const cplx<T> Bv = has2 ? cscale(src[(int64_t)(fa + 1) * a.fsize + k1], gk) : zero;
-        cplx<T>* row = Z + ty * PL;
-        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
-        if (k1 > 0 && k1 < H) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int id = lane + 64 * i;
+        if (id < 2 * (H + 1)) {
+          const int ty = id > H;
+          const int k1 = ty ? id - H - 1 : id;
+          const T gk = ty ? gv1[k1] : gm1[k1];
+          const cplx<T> A = cscale(la[i], gk);
+          const cplx<T> Bv = has2 ? cscale(lb[i], gk) : zero;
+          cplx<T>* row = Z + ty * PL;
+          row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+          if (k1 > 0 && k1 < H) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+        }
       }
       wave_sync();
 #pragma unroll
@@ -1225,6 +1249,7 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
         *(double*)(Z + PL + pad8(j)) = x1;
       }
     }
+    issue(b + 2 * kWaves1d);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -1285,20 +1310,42 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d_gen(VmrArgs<T> a) {
   const int j = tid;  // n1 <= 512 = NT: one point per thread
   double mean = 0.0, m2 = 0.0;
   const cplx<T> zero = {(T)0, (T)0};
+  // loads of a round in registers, one round ahead (as k_ladder_1d); the
+  // half spectrum has 2 H1 <= 2 * 257 items over the wave's 64 lanes
+  constexpr int NIT = (2 * 257 + 63) / 64;
+  cplx<T> la[NIT], lb[NIT];
+  auto issue = [&](int b) {
+    const int bb = b < B ? b : 0;
+    const int o2 = bb + 1 < B ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int id = lane + 64 * i;
+      const int idc = id < 2 * H1 ? id : 0;
+      const int ty = idc >= H1;
+      const int k1 = ty ? idc - H1 : idc;
+      const int fa = ty * B + bb;
+      la[i] = src[(int64_t)fa * a.fsize + k1];
+      lb[i] = src[(int64_t)(fa + o2) * a.fsize + k1];
+    }
+  };
+  issue(2 * wv);
   for (int b0 = 0; b0 < B; b0 += 2 * kWaves1d) {
     const int b = b0 + 2 * wv;
     if (b < B) {
       const bool has2 = b + 1 < B;
-      for (int id = lane; id < 2 * H1; id += 64) {
-        const int ty = id >= H1;
-        const int k1 = ty ? id - H1 : id;
-        const int fa = ty * B + b;
-        const T gk = ty ? gv1[k1] : gm1[k1];
-        const cplx<T> A = cscale(src[(int64_t)fa * a.fsize + k1], gk);
-        const cplx<T> Bv = has2 ? cscale(src[(int64_t)(fa + 1) * a.fsize + k1], gk) : zero;
-        cplx<T>* row = Z + ty * PL;
-        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
-        if (k1 > 0 && 2 * k1 < n1) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int id = lane + 64 * i;
+        if (id < 2 * H1) {
+          const int ty = id >= H1;
+          const int k1 = ty ? id - H1 : id;
+          const T gk = ty ? gv1[k1] : gm1[k1];
+          const cplx<T> A = cscale(la[i], gk);
+          const cplx<T> Bv = has2 ? cscale(lb[i], gk) : zero;
+          cplx<T>* row = Z + ty * PL;
+          row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
+          if (k1 > 0 && 2 * k1 < n1) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+        }
       }
       wave_sync();
       wave_fft_gen<1, T, 64>(Z, lane, true, a.fac, twl);
@@ -1313,6 +1360,7 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d_gen(VmrArgs<T> a) {
         *(double*)(Z + PL + pad8(jj)) = x1;
       }
     }
+    issue(b + 2 * kWaves1d);
     __syncthreads();
     if (j < n1) {
       for (int w = 0; w < kWaves1d; ++w) {
