@@ -10,7 +10,9 @@ A step is one whole estimate (binning, forward FFTs, the 251-rung ladder,
 convergence, density) on device-resident samples.  `value` = estimates/s.
 
 Other workloads (--workload): c1 (1-D 512 grid, 10k N(0,1)), c3 (3-D 128^3,
-1e6 correlated Gaussian + 10 % uniform), batch (64 heterogeneous datasets,
+1e6 correlated Gaussian + 10 % uniform), 3d64 (a 64^3 dataset of the kind the
+batched mix holds: 2e5 correlated Gaussian + 10 % uniform over [-6,6]^3),
+batch (64 heterogeneous datasets,
 batched vs plain), serve (Poisson stream through the continuous-batching
 server: p50/p99 latency vs the plain per-request path).
 
@@ -50,6 +52,10 @@ def workload(name):
         return dict(x=W.correlated_3d(1000000, 3), grid=[(-4.0, 4.0, 128)] * 3, B=100,
                     desc="3-D correlated Gaussian (rho=0.5) + 10% uniform, n=1e6, 128^3 grid over "
                          "[-4,4]^3, B=100, S=251")
+    if name == "3d64":
+        return dict(x=W.correlated_3d(200000, 4, lo=-6.0, hi=6.0), grid=[(-6.0, 6.0, 64)] * 3,
+                    B=100, desc="3-D correlated Gaussian (rho=0.5) + 10% uniform, n=2e5, 64^3 grid "
+                                "over [-6,6]^3, B=100, S=251")
     raise SystemExit(f"unknown workload {name}")
 
 
