@@ -121,18 +121,29 @@ static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; 
 static bool use_slab3(const pkde_plan* pl) { return pl->prec && slab3_ok(pl->g); }
 static int y_units(const pkde_plan* pl) { return pl->g.N == 3 && !use_slab3(pl) ? 2 : 1; }
 
-// Rungs per chunk: as many as the chunk budget holds, and -- unless the
-// budget was set explicitly -- at least 4 when they fit in 1/8 of the device
-// memory (the first axis pass reads the spectra once per chunk).
+// Rungs per chunk: as many as the chunk budget holds (8 GiB, or 1/32 of the
+// device memory if that is less), and -- unless the budget was set
+// explicitly -- at least 4 when they fit in 1/8 of the device memory; then
+// spread evenly over the chunks the ladder needs, so the last chunk is not a
+// small remainder.  Every chunk pays the tails of its kernels (the waves of
+// its last, partly filled round) and reads the spectra once: at 256^2 the
+// ladder takes 11.97 ms with 4 chunks against 12.38 ms with 7, at 64^3
+// 66.5 ms with 13 against 74.8 ms with 26.
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
   const uint64_t units = (uint64_t)y_units(pl);
   if (per_rung == 0) return (uint64_t)fsize(pl) * 2 * rs * 2;  // :rot: the means pass only
-  uint64_t rungs = std::max<uint64_t>(1, pl->ctx->chunk_bytes / units / per_rung);
+  uint64_t budget = pl->ctx->chunk_bytes;
+  if (!pl->ctx->chunk_fixed && pl->ctx->mem_total)
+    budget = std::min<uint64_t>(budget, pl->ctx->mem_total / 32);
+  uint64_t rungs = std::max<uint64_t>(1, budget / units / per_rung);
   if (!pl->ctx->chunk_fixed)
     rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
-  rungs = std::min<uint64_t>(rungs, (uint64_t)pl->S);
+  const uint64_t S = (uint64_t)pl->S;
+  rungs = std::min<uint64_t>(rungs, S);
+  const uint64_t chunks = (S + rungs - 1) / rungs;
+  rungs = (S + chunks - 1) / chunks;
   return std::max<uint64_t>(rungs * per_rung, (uint64_t)fsize(pl) * 2 * rs * 2);
 }
 

@@ -26,7 +26,7 @@ struct pkde_ctx {
   double* tw64 = nullptr;  // exp(-2 pi i k / L) for L = 2..4096 at offset L-2
   float* tw32 = nullptr;
   std::mutex mu;
-  uint64_t chunk_bytes = 4ull << 30;  // per-plan budget of the rung-chunk buffer
+  uint64_t chunk_bytes = 8ull << 30;  // per-plan budget of the rung-chunk buffer
   bool chunk_fixed = false;           // budget set by PKDE_CHUNK_BYTES
   uint64_t mem_total = 0;             // device memory, bytes
   // batched form
