@@ -33,6 +33,37 @@ def test_library_exports_header_symbols():
     assert dims == _lib.MAX_DIMS  # the ctypes structs' per-axis arrays
 
 
+def test_ctypes_structs_match_header(tmp_path):
+    """The Python mirror of the ABI structs has the C layout: sizes and field
+    offsets printed by a C program compiled against the header."""
+    import shutil
+    import subprocess
+    from parallelkde_rocm import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    checks = [("pkde_gradepro_params", _lib.Params, ["bw_final", "alpha_s", "rng_seed", "precision"]),
+              ("pkde_stats", _lib.Stats, ["eps", "bw_final", "ms_total", "algorithmic_bytes"]),
+              ("pkde_request", _lib.Request, ["n_per_axis", "h", "samples", "params", "chosen_out"]),
+              ("pkde_server_cfg", _lib.ServerCfg, ["memory_budget"])]
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, _, fields in checks:
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        lines += [f'  printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "abi"
+    subprocess.run([cc, "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = []
+    for _, cls, fields in checks:
+        want.append(C.sizeof(cls))
+        want += [getattr(cls, f).offset for f in fields]
+    assert got == want
+
+
 def test_params_default_matches_reference_defaults():
     from parallelkde_rocm import _lib
     p = _lib.Params()
