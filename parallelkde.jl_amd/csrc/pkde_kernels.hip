@@ -1771,18 +1771,24 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // ------------------------------------------------ 3-D ladder, small slabs
 // The 3-D ladder step after the axis-2 pass for grids whose (k1, k3) slab of
 // a field pair fits LDS (n1 n3 up to 64 x 64, 128 x 32 or 256 x 16 in FP64):
-// one workgroup of 16 waves per (rung, j2) walks the bootstraps in order.
+// one workgroup of 8 waves per (rung, j2) walks the bootstraps in order.
 // Per bootstrap b: the (k1, k3) planes of column j2 of its sums and squares
 // fields ([k1][j2][k3] from the axis-2 pass, k3 contiguous) go to LDS with
 // their axis-3 Gaussian factors, their axis-3 inverse transform runs
-// wave-locally on the 2 n1/2 pencils, then each wave builds the axis-1 rows
-// of its n3/16 values of j3 with the sums field in the real part and the
-// squares field in the imaginary part (each with its axis-1 factor; Nyquist
-// column packed into column 0, as k_ladder_rows), transforms them, and
-// forms the VMR ratio and Welford update of each point.  Against the axis-3
-// pass into tiled rows followed by k_ladder_rows, the rung's fields are read
-// once and not written again.
-constexpr int kSlabThreads = 1024;
+// wave-locally on the 2 n1/2 pencils, then each wave gathers the values of
+// its n3/8 values of j3 into registers and writes them as axis-1 rows with
+// the sums field in the real part and the squares field in the imaginary
+// part (each with its axis-1 factor; Nyquist column packed into column 0,
+// as k_ladder_rows), transforms them, and forms the VMR ratio and Welford
+// update of each point.  Against the axis-3 pass into tiled rows followed by
+// k_ladder_rows, the rung's fields are read once and not written again.
+// Eight waves keep every lane of both transforms busy (64 x 64: 64 pencils of
+// 8 lanes, 8 rows per wave); at 176 VGPRs one workgroup runs per CU.  64^3,
+// B = 100: 3.47 -> 2.8 ms per 19-rung chunk against 16 waves with half the
+// lanes idle; two 4-wave workgroups per CU (planes and rows sharing LDS)
+// measured the same, and capping at 128 VGPRs for two 8-wave workgroups
+// spilled and ran slower.
+constexpr int kSlabThreads = 512;
 
 template <int LOG2N1, int LOG2N3>
 __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
@@ -1792,13 +1798,14 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   constexpr int NW = kSlabThreads / 64;
   constexpr int RW = n3 / NW;            // rows (values of j3) per wave
   constexpr int PPT = RW * n1 / 64;      // points per lane
+  constexpr int GPL = (RW * H + 63) / 64;  // plane values gathered per lane and plane
   constexpr int lp1 = n1 / E, ppc1 = 64 / lp1;
   constexpr int lp3 = n3 / E, ppc3 = 64 / lp3;
   static_assert(RW >= 1 && PPT >= 1 && RW * n1 == 64 * PPT, "slab shape");
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<double>* P = (cplx<double>*)smem;        // [2][H][PL3]: the pair's planes
-  cplx<double>* Zall = P + 2 * H * PL3;          // [NW][RW][PL1]: axis-1 rows
-  cplx<double>* twl1 = Zall + NW * RW * PL1;
+  cplx<double>* P = (cplx<double>*)smem;  // [2][H][PL3]: the pair's planes
+  cplx<double>* Zall = P + 2 * H * PL3;   // [NW][RW][PL1]: axis-1 rows
+  cplx<double>* twl1 = Zall + n3 * PL1;
   cplx<double>* twl3 = twl1 + n1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int k = tid; k < n1; k += kSlabThreads) twl1[k] = a.tw[n1 - 2 + k];
@@ -1839,6 +1846,22 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
       nx[i] = src[(int64_t)(bb + p * B) * a.fsize + (int64_t)k1 * a.Q + k3];
     }
   };
+  // axis-3 factors of this thread's plane values and axis-1 factors of its
+  // gathered row values: the same for every bootstrap
+  double sf[LPT], fa[GPL], fb[GPL];
+#pragma unroll
+  for (int i = 0; i < LPT; ++i) {
+    int p, k1, k3;
+    plane_elem(i, p, k1, k3);
+    sf[i] = (p ? gv : gm)[a.goff3 + k3];
+  }
+#pragma unroll
+  for (int i = 0; i < GPL; ++i) {
+    const int k1 = (lane + 64 * i) & (H - 1);
+    fa[i] = gm[k1];
+    fb[i] = gv[k1];
+  }
+  const double g0m = gm[0], gNm = gm[H], g0v = gv[0], gNv = gv[H];
   if constexpr (kPrefetch) load_planes(0);
   for (int b = 0; b < B; ++b) {
     const double inv = a.invk[b];
@@ -1848,7 +1871,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     for (int i = 0; i < LPT; ++i) {
       int p, k1, k3;
       plane_elem(i, p, k1, k3);
-      P[(p * H + k1) * PL3 + pad8(k3)] = cscale(nx[i], (p ? gv : gm)[a.goff3 + k3]);
+      P[(p * H + k1) * PL3 + pad8(k3)] = cscale(nx[i], sf[i]);
     }
     if constexpr (kPrefetch) load_planes(b + 1 < B ? b + 1 : b);  // in flight meanwhile
     __syncthreads();
@@ -1858,19 +1881,29 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
       wave_fft<LOG2N3, 1, double, E>(P + (act ? pen : 0) * PL3, lane % lp3, act, twl3);
     }
     __syncthreads();
-    for (int id = lane; id < RW * H; id += 64) {
-      const int w = id >> (LOG2N1 - 1), k1 = id & (H - 1);
+    // this wave's rows j3 in [wv RW, wv RW + RW) of both planes, all k1
+    cplx<double> ga[GPL], gb[GPL];
+#pragma unroll
+    for (int i = 0; i < GPL; ++i) {
+      const int id = lane + 64 * i;
+      const int w = (id >> (LOG2N1 - 1)) & (RW - 1), k1 = id & (H - 1);
       const int j3 = wv * RW + w;
-      const cplx<double> pa = P[k1 * PL3 + pad8(j3)];
-      const cplx<double> pb = P[(H + k1) * PL3 + pad8(j3)];
+      ga[i] = P[k1 * PL3 + pad8(j3)];
+      gb[i] = P[(H + k1) * PL3 + pad8(j3)];
+    }
+#pragma unroll
+    for (int i = 0; i < GPL; ++i) {
+      const int id = lane + 64 * i;
+      if (RW * H % 64 != 0 && id >= RW * H) break;
+      const int w = id >> (LOG2N1 - 1), k1 = id & (H - 1);
+      const cplx<double> pa = ga[i], pb = gb[i];
       cplx<double>* row = Z + w * PL1;
       if (k1 == 0) {
         // packed column: real part k1 = 0, imaginary part k1 = n1/2
-        const double g0m = gm[0], gNm = gm[H], g0v = gv[0], gNv = gv[H];
         row[pad8(0)] = {pa.re * g0m, pb.re * g0v};
         row[pad8(H)] = {pa.im * gNm, pb.im * gNv};
       } else {
-        const cplx<double> A = cscale(pa, gm[k1]), Bv = cscale(pb, gv[k1]);
+        const cplx<double> A = cscale(pa, fa[i]), Bv = cscale(pb, fb[i]);
         row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
         row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
       }
@@ -1892,7 +1925,6 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
       mean[u] += d * inv;
       m2[u] += d * (xv - mean[u]);
     }
-    wave_sync();
   }
   const double sc = a.vscale[r];
 #pragma unroll
@@ -1906,9 +1938,8 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
 }
 
 static size_t slab3_lds(int n1, int n3) {
-  const int H = n1 / 2, RW = n3 / (kSlabThreads / 64);
-  return ((size_t)2 * H * pencil_stride(n3) + (size_t)(kSlabThreads / 64) * RW * pencil_stride(n1) +
-          n1 + n3) *
+  // planes [2][n1/2][PL3], rows [n3][PL1], twiddles
+  return ((size_t)n1 * pencil_stride(n3) + (size_t)n3 * pencil_stride(n1) + n1 + n3) *
          sizeof(cplx<double>);
 }
 

@@ -260,6 +260,10 @@ CASES = [
     ("3d-64x16x64", ([-4.0] * 3, [4.0] * 3, [64, 16, 64]), lambda: W.correlated_3d(20000, 33), 20, 60),
     ("3d-128x16x32", ([-4.0] * 3, [4.0] * 3, [128, 16, 32]), lambda: W.correlated_3d(20000, 34), 20,
      60),
+    ("3d-256x8x16", ([-4.0] * 3, [4.0] * 3, [256, 8, 16]), lambda: W.correlated_3d(20000, 38), 20,
+     60),
+    ("3d-16x32x64", ([-4.0] * 3, [4.0] * 3, [16, 32, 64]), lambda: W.correlated_3d(20000, 39), 20,
+     60),
     # more than three dimensions (the reference's CPU path takes any N)
     ("4d-16x16x8x8", ([-4.0] * 4, [4.0] * 4, [16, 16, 8, 8]), lambda: W.gaussian_nd(20000, 4, 28),
      20, 60),
