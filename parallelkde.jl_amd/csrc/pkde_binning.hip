@@ -193,8 +193,16 @@ struct TileGeom {
 };
 
 // Deposits of bootstrap b (b == B: the full sample, multiplicity 1) into one
-// grid tile.  Corner weights and fixed-point values follow k_binning
-// operation for operation.
+// grid tile, N = 1, 2 or 3.  Corner weights and fixed-point values follow
+// k_binning operation for operation.  Each thread takes kBatch samples of the
+// tile's sorted run per trip and issues all their loads (sorted cell,
+// remainders, sample index, then the multiplicities through the sample
+// indices) before depositing any of them, so that one trip waits for one
+// round of memory latency rather than kBatch: with one 128 KB workgroup per
+// CU the loop is latency-bound.
+constexpr int kBatch = 4;
+
+template <int N>
 __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, TileGeom tg,
                                                              const int32_t* __restrict__ start,
                                                              const int32_t* __restrict__ scell,
@@ -211,7 +219,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, T
   const int b = blockIdx.x / ntiles;
   const int tile = blockIdx.x - b * ntiles;
   const int tyi = tile % tg.tiles_y, tzi = tile / tg.tiles_y;
-  const int N = g.N;
   const int n1 = g.n[0], n2 = N >= 2 ? g.n[1] : 1;
   const int y0 = tyi * tg.ty, z0 = tzi * tg.tz;
   const int T = tg.T;
@@ -227,56 +234,74 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, T
     const int zhi = N == 3 ? std::min(z0 + tg.tz, n3) : 1;  // exclusive
     const int ylo = N >= 2 ? std::max(0, y0 - 1) : 0;
     const int yhi = N >= 2 ? std::min(y0 + tg.ty, n2) : 1;
+    const uint8_t* cntb = cnt + (int64_t)b * npad;
     for (int z = zlo; z < zhi; ++z) {
       // base cells of rows [ylo, yhi) of plane z: one contiguous run
       const int64_t cA = ((int64_t)z * n2 + ylo) * n1;
       const int64_t cB = ((int64_t)z * n2 + yhi) * n1;
       const int32_t pA = start[cA], pB = start[std::min<int64_t>(cB, g.G)];
-      for (int32_t pos = pA + threadIdx.x; pos < pB; pos += kBinThreads) {
-        const unsigned c = sq ? (unsigned)cnt[(int64_t)b * npad + order[pos]] : 1u;
-        if (c == 0u) continue;
-        const int32_t cl = scell[pos];
-        int il[kMaxDims];
-        il[0] = cl % n1;
-        if (N >= 2) il[1] = (cl / n1) % n2;
-        if (N == 3) il[2] = cl / (n1 * n2);
-        double rl[kMaxDims], rh[kMaxDims];
-        for (int i = 0; i < N; ++i) {
-          rh[i] = srh[(int64_t)pos * N + i];
-          rl[i] = g.h[i] - rh[i];
+      for (int32_t p0 = pA + threadIdx.x; p0 < pB; p0 += kBatch * kBinThreads) {
+        int32_t cl[kBatch], ord[kBatch];
+        double rh[kBatch][N];
+        unsigned c[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int32_t pos = p0 + u * kBinThreads;
+          const bool in = pos < pB;
+          cl[u] = in ? scell[pos] : 0;
+          ord[u] = in && sq ? order[pos] : -1;
+#pragma unroll
+          for (int i = 0; i < N; ++i) rh[u][i] = in ? srh[(int64_t)pos * N + i] : 0.0;
+          c[u] = in ? 1u : 0u;
         }
-        for (int cc = 0; cc < (1 << N); ++cc) {
-          double pl = 1.0, ph = 1.0;
-          bool hl = false, hh = false;
-          int li[kMaxDims];
-          for (int j = 0; j < N; ++j) {
-            const int mask = (cc >> (N - 1 - j)) & 1;
-            li[j] = mask ? il[j] : il[j] + 1;
-            if (mask) {
-              pl = hl ? pl * rl[j] : rl[j];
-              hl = true;
-            } else {
-              ph = hh ? ph * rh[j] : rh[j];
-              hh = true;
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+          if (ord[u] >= 0) c[u] = (unsigned)cntb[ord[u]];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          if (c[u] == 0u) continue;
+          int il[N];
+          il[0] = cl[u] % n1;
+          if (N >= 2) il[1] = (cl[u] / n1) % n2;
+          if (N == 3) il[2] = cl[u] / (n1 * n2);
+          double rl[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) rl[i] = g.h[i] - rh[u][i];
+#pragma unroll
+          for (int cc = 0; cc < (1 << N); ++cc) {
+            double pl = 1.0, ph = 1.0;
+            bool hl = false, hh = false;
+            int li[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+              const int mask = (cc >> (N - 1 - j)) & 1;
+              li[j] = mask ? il[j] : il[j] + 1;
+              if (mask) {
+                pl = hl ? pl * rl[j] : rl[j];
+                hl = true;
+              } else {
+                ph = hh ? ph * rh[u][j] : rh[u][j];
+                hh = true;
+              }
             }
+            // local cell in the tile (axis 1 always inside)
+            int loc = li[0];
+            if (N >= 2) {
+              const int yy = li[1] - y0;
+              if (yy < 0 || yy >= tg.ty) continue;
+              loc += yy * n1;
+            }
+            if (N == 3) {
+              const int zz = li[2] - z0;
+              if (zz < 0 || zz >= tg.tz) continue;
+              loc += zz * n1 * tg.ty;
+            }
+            const double term = (pl * ph) / denom;
+            atomicAdd(lM + loc, (unsigned long long)c[u] * (unsigned long long)llrint(term * scaleM));
+            if (sq)
+              atomicAdd(lV + loc,
+                        (unsigned long long)c[u] * (unsigned long long)llrint((term * term) * scaleV));
           }
-          // local cell in the tile (axis 1 always inside)
-          int loc = li[0];
-          if (N >= 2) {
-            const int yy = li[1] - y0;
-            if (yy < 0 || yy >= tg.ty) continue;
-            loc += yy * n1;
-          }
-          if (N == 3) {
-            const int zz = li[2] - z0;
-            if (zz < 0 || zz >= tg.tz) continue;
-            loc += zz * n1 * tg.ty;
-          }
-          const double term = (pl * ph) / denom;
-          atomicAdd(lM + loc, (unsigned long long)c * (unsigned long long)llrint(term * scaleM));
-          if (sq)
-            atomicAdd(lV + loc,
-                      (unsigned long long)c * (unsigned long long)llrint((term * term) * scaleV));
         }
       }
     }
@@ -461,9 +486,11 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
   static const bool attrs = [] {
     (void)hipFuncSetAttribute((const void*)k_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
                               kCountChunk);
-    (void)hipFuncSetAttribute((const void*)k_bin_deposit,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              2 * kTileCells * (int)sizeof(unsigned long long));
+    const void* dep[3] = {(const void*)k_bin_deposit<1>, (const void*)k_bin_deposit<2>,
+                          (const void*)k_bin_deposit<3>};
+    for (const void* f : dep)
+      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                2 * kTileCells * (int)sizeof(unsigned long long));
     return true;
   }();
   (void)attrs;
@@ -487,8 +514,9 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
   const TileGeom tg = tile_geom(g);
   const unsigned nblk = (unsigned)((B + 1) * tg.tiles_y * tg.tiles_z);
   const size_t lds = (size_t)2 * tg.T * sizeof(unsigned long long);
-  k_bin_deposit<<<nblk, kBinThreads, lds, s>>>(g, B, tg, start, scell, order, srh, cnt, npad, ovf,
-                                               denom, scaleM, scaleV, (unsigned long long*)acc);
+  auto deposit = g.N == 1 ? k_bin_deposit<1> : g.N == 2 ? k_bin_deposit<2> : k_bin_deposit<3>;
+  deposit<<<nblk, kBinThreads, lds, s>>>(g, B, tg, start, scell, order, srh, cnt, npad, ovf, denom,
+                                         scaleM, scaleV, (unsigned long long*)acc);
   if (B > 0)
     k_bin_fallback<<<(unsigned)B, kThreads, 0, s>>>(g, x, n, idx, B, key0, key1, ovf, denom,
                                                     scaleM, scaleV, (unsigned long long*)acc);
