@@ -450,6 +450,68 @@ __device__ __forceinline__ void wave_fft_from(cplx<T>* pb, int t, bool active,
   }
 }
 
+// The first pass (radix E, Ns = 1) of a wave-local transform from registers:
+// v[r] = x[t + r L/E] (the inputs wave_pass would read); the results go to
+// the pencil in LDS as wave_pass stores them.  The caller continues with
+// wave_fft_from<LOG2L, log2 E>.  Needs L > E.
+template <int LOG2L, int E, int DIR, typename T>
+__device__ __forceinline__ void wave_first_pass(cplx<T>* pb, int t, bool active, cplx<T>* v) {
+  constexpr int LE = ilog2c(E);
+  static_assert((1 << LOG2L) > E, "first pass from registers needs L > E");
+  if (active) {
+    dft_reg<E, DIR, T>(v);
+#pragma unroll
+    for (int r = 0; r < E; ++r) pb[pad8((t << LE) + r)] = v[r];
+  }
+  wave_sync();
+}
+
+// The last pass (Ns R = L) of a wave-local transform, its outputs kept in
+// registers: out[u R + r] = X[j + r Ns] with j = t + u L/E.
+template <int R, int E, int LOG2L, int LOG2NS, int DIR, typename T>
+__device__ __forceinline__ void wave_last_pass_regs(const cplx<T>* pb, int t, bool active,
+                                                    const cplx<T>* __restrict__ tw, cplx<T>* out) {
+  constexpr int L = 1 << LOG2L;
+  constexpr int tpp = L / E;
+  constexpr int nbfly = L / R;
+  constexpr int Ns = 1 << LOG2NS;
+  static_assert(Ns * R == L, "not the last pass");
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = t + u * tpp;
+      cplx<T>* w = out + u * R;
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = pb[pad8(j + r * nbfly)];
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> f = tw[j * r];
+          if (DIR > 0) f.im = -f.im;
+          w[r] = cmul(w[r], f);
+        }
+      }
+      dft_reg<R, DIR, T>(w);
+    }
+  }
+}
+
+// Passes LOG2NS.. of a wave-local transform, the last one into registers
+// (wave_last_pass_regs).  Needs LOG2L > LOG2NS.
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T>
+__device__ __forceinline__ void wave_fft_to_regs(cplx<T>* pb, int t, bool active,
+                                                 const cplx<T>* __restrict__ tw, cplx<T>* out) {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  static_assert(rem > 0, "no pass left");
+  if constexpr (rem > LE) {
+    wave_pass<E, E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw);
+    wave_fft_to_regs<LOG2L, LOG2NS + LE, E, DIR, T>(pb, t, active, tw, out);
+  } else {
+    wave_last_pass_regs<(1 << rem), E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw, out);
+  }
+}
+
 // Wave-local FFT of the lane's pencil (padded layout): L = 2^LOG2L with
 // E <= L <= 64 * E, so the L/E threads of a pencil sit in one wavefront.
 // E = 16 gives radix-16 passes; E = 8 (radix-8 passes) halves the registers

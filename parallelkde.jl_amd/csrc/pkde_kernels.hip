@@ -1772,22 +1772,23 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // The 3-D ladder step after the axis-2 pass for grids whose (k1, k3) slab of
 // a field pair fits LDS (n1 n3 up to 64 x 64, 128 x 32 or 256 x 16 in FP64):
 // one workgroup of 8 waves per (rung, j2) walks the bootstraps in order.
-// Per bootstrap b: the (k1, k3) planes of column j2 of its sums and squares
-// fields ([k1][j2][k3] from the axis-2 pass, k3 contiguous) go to LDS with
-// their axis-3 Gaussian factors, their axis-3 inverse transform runs
-// wave-locally on the 2 n1/2 pencils, then each wave gathers the values of
-// its n3/8 values of j3 into registers and writes them as axis-1 rows with
-// the sums field in the real part and the squares field in the imaginary
-// part (each with its axis-1 factor; Nyquist column packed into column 0,
-// as k_ladder_rows), transforms them, and forms the VMR ratio and Welford
-// update of each point.  Against the axis-3 pass into tiled rows followed by
-// k_ladder_rows, the rung's fields are read once and not written again.
-// Eight waves keep every lane of both transforms busy (64 x 64: 64 pencils of
-// 8 lanes, 8 rows per wave); at 176 VGPRs one workgroup runs per CU.  64^3,
-// B = 100: 3.47 -> 2.8 ms per 19-rung chunk against 16 waves with half the
-// lanes idle; two 4-wave workgroups per CU (planes and rows sharing LDS)
-// measured the same, and capping at 128 VGPRs for two 8-wave workgroups
-// spilled and ran slower.
+// Per bootstrap b, for column j2 of its sums and squares fields ([k1][j2][k3]
+// from the axis-2 pass, k3 contiguous):
+//  - axis 3: the lanes of the pencil (plane, k1) load exactly the inputs of
+//    their first radix-8 pass, one bootstrap ahead, scale them by the axis-3
+//    factors and run that pass from registers; the other passes go through
+//    the pencil in LDS;
+//  - axis 1: the lanes of row j3 read, per input of their first radix-8
+//    pass, the pair (sums, squares) at the column that input comes from and
+//    form the row with the sums field in the real part and the squares
+//    field in the imaginary part (each with its axis-1 factor; Nyquist
+//    column packed into column 0, as k_ladder_rows); the first pass runs
+//    from registers and the last one leaves its outputs in registers, where
+//    the VMR ratio and Welford update of each point are formed.
+// Against the axis-3 pass into tiled rows followed by k_ladder_rows, the
+// rung's fields are read once and not written again.  Eight waves keep every
+// lane of both transforms busy at 64 x 64 (64 pencils of 8 lanes, 8 rows of
+// 8 lanes per wave); one workgroup runs per CU.
 constexpr int kSlabThreads = 512;
 
 template <int LOG2N1, int LOG2N3>
@@ -1796,15 +1797,18 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   constexpr int E = 8;
   constexpr int PL1 = pencil_stride(n1), PL3 = pencil_stride(n3);
   constexpr int NW = kSlabThreads / 64;
-  constexpr int RW = n3 / NW;            // rows (values of j3) per wave
-  constexpr int PPT = RW * n1 / 64;      // points per lane
-  constexpr int GPL = (RW * H + 63) / 64;  // plane values gathered per lane and plane
-  constexpr int lp1 = n1 / E, ppc1 = 64 / lp1;
-  constexpr int lp3 = n3 / E, ppc3 = 64 / lp3;
-  static_assert(RW >= 1 && PPT >= 1 && RW * n1 == 64 * PPT, "slab shape");
+  constexpr int RW = n3 / NW;                  // rows (values of j3) per wave
+  constexpr int lp1 = n1 / E, lp3 = n3 / E;    // lanes per row / per pencil
+  constexpr int nb1 = n1 / E, nb3 = n3 / E;    // input stride of the first passes
+  constexpr int ppc3 = 64 / lp3;               // pencils per wave
+  // the last axis-1 pass: radix and sub-transform length
+  constexpr int LR1 = (LOG2N1 - 3) % 3 == 0 ? 8 : (1 << ((LOG2N1 - 3) % 3));
+  constexpr int NS1 = n1 / LR1;
+  static_assert(RW >= 1 && RW * lp1 <= 64 && NW * ppc3 >= 2 * H, "slab shape");
+  static_assert(n1 > E && n3 > E, "slab axes longer than one radix-8 pass");
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<double>* P = (cplx<double>*)smem;  // [2][H][PL3]: the pair's planes
-  cplx<double>* Zall = P + 2 * H * PL3;   // [NW][RW][PL1]: axis-1 rows
+  cplx<double>* P = (cplx<double>*)smem;       // [2][H][PL3]: the pair's planes
+  cplx<double>* Zall = P + 2 * H * PL3;        // [NW][RW][PL1]: axis-1 rows
   cplx<double>* twl1 = Zall + n3 * PL1;
   cplx<double>* twl3 = twl1 + n1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1816,124 +1820,105 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   const double* gm = a.gM + (int64_t)r * a.ksum;
   const double* gv = a.gV + (int64_t)r * a.ksum;
   const double vfac = a.cV[r] * a.invG;
-  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * n3;
-  cplx<double>* Z = Zall + wv * RW * PL1;
-  double mean[PPT], m2[PPT];
+  const int B = a.B;
+  // axis-3 pencil of this lane: plane pl (0 sums field b, 1 squares field
+  // B + b), column k1; its thread t3
+  const int pen = wv * ppc3 + lane / lp3, t3 = lane % lp3;
+  const bool act3 = pen < 2 * H;
+  const int pl = act3 ? pen / H : 0, k1p = act3 ? pen % H : 0;
+  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * n3 +
+                            (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q + t3;
+  cplx<double>* pb3 = P + (act3 ? pen : 0) * PL3;
+  double sf[E];
 #pragma unroll
-  for (int u = 0; u < PPT; ++u) {
+  for (int i = 0; i < E; ++i) sf[i] = (pl ? gv : gm)[a.goff3 + t3 + i * nb3];
+  cplx<double> nx[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) nx[i] = {0.0, 0.0};
+  auto load = [&](int bb) {
+    if (act3) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) nx[i] = src[(int64_t)bb * a.fsize + i * nb3];
+    }
+  };
+  // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
+  // input i (= t1 + i nb1) and its factors
+  const int w = lane / lp1, t1 = lane % lp1;
+  const bool act1 = w < RW;
+  const int j3 = wv * RW + (act1 ? w : 0);
+  cplx<double>* pb1 = Zall + (wv * RW + (act1 ? w : 0)) * PL1;
+  int col[E];
+  double fm[E], fv[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int k = t1 + i * nb1;
+    col[i] = k < H ? k : (k == H ? 0 : n1 - k);
+    fm[i] = gm[col[i]];
+    fv[i] = gv[col[i]];
+  }
+  if (t1 == 0) {
+    // inputs 0 and E/2 are columns 0 and n1/2, packed in column 0
+    fm[0] = gm[0];
+    fv[0] = gv[0];
+    fm[E / 2] = gm[H];
+    fv[E / 2] = gv[H];
+  }
+  double mean[E], m2[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
     mean[u] = 0.0;
     m2[u] = 0.0;
   }
-  const int B = a.B;
-  // plane values per thread; rows up to 64 points also hold the next
-  // bootstrap's planes in registers while this one is transformed (the
-  // longer rows would spill)
-  constexpr int LPT = 2 * H * n3 / kSlabThreads;
-  constexpr bool kPrefetch = LOG2N1 <= 6;
-  cplx<double> nx[LPT];
-  auto plane_elem = [&](int i, int& p, int& k1, int& k3) {
-    const int id = tid + i * kSlabThreads;
-    p = id >> (LOG2N1 - 1 + LOG2N3);
-    k1 = (id >> LOG2N3) & (H - 1);
-    k3 = id & (n3 - 1);
-  };
-  // plane 0: sums field bb (means factors); plane 1: squares field B + bb
-  auto load_planes = [&](int bb) {
-#pragma unroll
-    for (int i = 0; i < LPT; ++i) {
-      int p, k1, k3;
-      plane_elem(i, p, k1, k3);
-      nx[i] = src[(int64_t)(bb + p * B) * a.fsize + (int64_t)k1 * a.Q + k3];
-    }
-  };
-  // axis-3 factors of this thread's plane values and axis-1 factors of its
-  // gathered row values: the same for every bootstrap
-  double sf[LPT], fa[GPL], fb[GPL];
-#pragma unroll
-  for (int i = 0; i < LPT; ++i) {
-    int p, k1, k3;
-    plane_elem(i, p, k1, k3);
-    sf[i] = (p ? gv : gm)[a.goff3 + k3];
-  }
-#pragma unroll
-  for (int i = 0; i < GPL; ++i) {
-    const int k1 = (lane + 64 * i) & (H - 1);
-    fa[i] = gm[k1];
-    fb[i] = gv[k1];
-  }
-  const double g0m = gm[0], gNm = gm[H], g0v = gv[0], gNv = gv[H];
-  if constexpr (kPrefetch) load_planes(0);
+  load(0);
   for (int b = 0; b < B; ++b) {
     const double inv = a.invk[b];
-    if constexpr (!kPrefetch) load_planes(b);
-    __syncthreads();  // every wave is done with the previous planes
+    cplx<double> v[E];
 #pragma unroll
-    for (int i = 0; i < LPT; ++i) {
-      int p, k1, k3;
-      plane_elem(i, p, k1, k3);
-      P[(p * H + k1) * PL3 + pad8(k3)] = cscale(nx[i], sf[i]);
-    }
-    if constexpr (kPrefetch) load_planes(b + 1 < B ? b + 1 : b);  // in flight meanwhile
+    for (int i = 0; i < E; ++i) v[i] = cscale(nx[i], sf[i]);
+    load(b + 1 < B ? b + 1 : b);  // in flight while this bootstrap is transformed
+    __syncthreads();  // every wave is done reading the previous planes
+    wave_first_pass<LOG2N3, E, 1, double>(pb3, t3, act3, v);
+    wave_fft_from<LOG2N3, 3, E, 1, double>(pb3, t3, act3, twl3);
     __syncthreads();
-    for (int c = wv * ppc3; c < 2 * H; c += NW * ppc3) {
-      const int pen = c + lane / lp3;
-      const bool act = pen < 2 * H;
-      wave_fft<LOG2N3, 1, double, E>(P + (act ? pen : 0) * PL3, lane % lp3, act, twl3);
-    }
-    __syncthreads();
-    // this wave's rows j3 in [wv RW, wv RW + RW) of both planes, all k1
-    cplx<double> ga[GPL], gb[GPL];
 #pragma unroll
-    for (int i = 0; i < GPL; ++i) {
-      const int id = lane + 64 * i;
-      const int w = (id >> (LOG2N1 - 1)) & (RW - 1), k1 = id & (H - 1);
-      const int j3 = wv * RW + w;
-      ga[i] = P[k1 * PL3 + pad8(j3)];
-      gb[i] = P[(H + k1) * PL3 + pad8(j3)];
+    for (int i = 0; i < E; ++i) {
+      const cplx<double> pa = P[col[i] * PL3 + pad8(j3)];
+      const cplx<double> pq = P[(H + col[i]) * PL3 + pad8(j3)];
+      const cplx<double> A = cscale(pa, fm[i]), Bv = cscale(pq, fv[i]);
+      if (i < E / 2)
+        v[i] = {A.re - Bv.im, A.im + Bv.re};  // column k = t1 + i nb1 < n1/2
+      else
+        v[i] = {A.re + Bv.im, Bv.re - A.im};  // column n1 - k
     }
+    if (t1 == 0) {
+      // packed column: real part k1 = 0, imaginary part k1 = n1/2
+      const cplx<double> pa = P[pad8(j3)], pq = P[H * PL3 + pad8(j3)];
+      v[0] = {pa.re * fm[0], pq.re * fv[0]};
+      v[E / 2] = {pa.im * fm[E / 2], pq.im * fv[E / 2]};
+    }
+    wave_first_pass<LOG2N1, E, 1, double>(pb1, t1, act1, v);
+    wave_fft_to_regs<LOG2N1, 3, E, 1, double>(pb1, t1, act1, twl1, v);
+    if (act1) {
 #pragma unroll
-    for (int i = 0; i < GPL; ++i) {
-      const int id = lane + 64 * i;
-      if (RW * H % 64 != 0 && id >= RW * H) break;
-      const int w = id >> (LOG2N1 - 1), k1 = id & (H - 1);
-      const cplx<double> pa = ga[i], pb = gb[i];
-      cplx<double>* row = Z + w * PL1;
-      if (k1 == 0) {
-        // packed column: real part k1 = 0, imaginary part k1 = n1/2
-        row[pad8(0)] = {pa.re * g0m, pb.re * g0v};
-        row[pad8(H)] = {pa.im * gNm, pb.im * gNv};
-      } else {
-        const cplx<double> A = cscale(pa, fa[i]), Bv = cscale(pb, fb[i]);
-        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
-        row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+      for (int u = 0; u < E; ++u) {
+        const cplx<double> z = v[u];  // sums + i squares
+        const double xv = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
+        const double d = xv - mean[u];
+        mean[u] += d * inv;
+        m2[u] += d * (xv - mean[u]);
       }
     }
-    wave_sync();
-#pragma unroll
-    for (int c = 0; c < RW; c += ppc1) {
-      const int pen = c + lane / lp1;
-      const bool act = pen < RW;
-      wave_fft<LOG2N1, 1, double, E>(Z + (act ? pen : 0) * PL1, lane % lp1, act, twl1);
-    }
-#pragma unroll
-    for (int u = 0; u < PPT; ++u) {
-      const int pt = lane + 64 * u;
-      const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
-      const cplx<double> z = Z[w * PL1 + pad8(j1)];  // sums + i squares
-      const double xv = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
-      const double d = xv - mean[u];
-      mean[u] += d * inv;
-      m2[u] += d * (xv - mean[u]);
-    }
   }
+  if (!act1) return;
   const double sc = a.vscale[r];
+  const int64_t q = (int64_t)j2 + (int64_t)n2 * j3;  // grid row j2 + n2 j3
+  double* out = a.vmr + (int64_t)r * a.G + q * n1;
 #pragma unroll
-  for (int u = 0; u < PPT; ++u) {
-    const int pt = lane + 64 * u;
-    const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
-    const int64_t q = (int64_t)j2 + (int64_t)n2 * (wv * RW + w);  // grid row j2 + n2 j3
+  for (int u = 0; u < E; ++u) {
+    // output u of the last pass: X[j + s NS1], j = t1 + (u / LR1) nb1, s = u % LR1
+    const int j1 = t1 + (u / LR1) * nb1 + (u % LR1) * NS1;
     const double vv = sc * m2[u] / (double)(B - 1);
-    a.vmr[(int64_t)r * a.G + q * n1 + j1] = isfinite(vv) ? log(vv) : (double)NAN;
+    out[j1] = isfinite(vv) ? log(vv) : (double)NAN;
   }
 }
 
