@@ -247,6 +247,15 @@ def run_batch(args):
 
     log("[bench] batch: warmup")
     plain_all()
+    # the ladder launch shape of each of the three grids, tuned on the device
+    # (pkde_gradepro_autotune); the batch's lane plans take it from the context
+    tuned = {}
+    for key, p in plans.items():
+        best, default = p.autotune(reps=2)
+        tuned["x".join(map(str, key))] = {"ladder_ms": round(best, 3),
+                                          "default_shape_ladder_ms": round(default, 3)}
+    log(f"[bench] batch: autotuned {tuned}")
+    plain_all()
     pk.estimate_batch(req)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -272,6 +281,7 @@ def run_batch(args):
         "plain_estimates_per_s": len(req) / t_plain,
         "batched_over_plain": t_plain / t_batch,
         "results_identical_to_plain": bool(same),
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": alg / t_batch / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": alg / t_batch / HBM_PEAK, "traffic": None,
                      "kernel": "whole batch, algorithmic ladder bytes U*(2B+1)*8 summed over "

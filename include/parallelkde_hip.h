@@ -166,6 +166,22 @@ int pkde_gradepro_last_stats(pkde_plan *plan, pkde_stats *stats);
  * rung, GradeProEstimator.jl:968-1051). */
 int pkde_gradepro_graph_enable(pkde_plan *plan, int enable);
 
+/* Autotuner of the ladder's launch shape for the plan's grid, B, S and
+ * precision.  Times the ladder of the plan's last estimate (its spectra stay
+ * resident; run one estimate first), `reps` times after one untimed run, for
+ * each candidate: rungs per chunk (the even splits of the ladder into the
+ * fewest chunks the chunk buffer allows and up to three more) times the axis
+ * pass's workgroup target (no split, 4000, 8000, 16000).  The fastest is
+ * kept for this plan and recorded in the context, where every plan of the
+ * same shape (the batched and serving forms included) picks it up.  Later
+ * estimates are bit-identical to untuned ones: the per-rung arithmetic does
+ * not depend on the launch shape.  best_ms_out / default_ms_out (nullable):
+ * ladder time of the choice and of the default shape.  No counterpart in the
+ * reference: its CUDA path sizes each launch with the occupancy heuristic
+ * (launch_configuration, src/StatisticsPropagation/FourierSpace.jl:285-287). */
+int pkde_gradepro_autotune(pkde_plan *plan, int32_t reps, double *best_ms_out,
+                           double *default_ms_out);
+
 /* Per-rung log-VMR table of the last estimate (S x G, rung-major), for
  * primitive-level parity (calculate_scaled_vmr!, DirectSpace.jl:526-584).
  * Only available when the plan was created with debug tables enabled. */

@@ -354,7 +354,8 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const bool slab = sizeof(T) == 8 && use_slab3(pl);  // 3-D: axes 3 and 1 fused
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
-    const int Rc = (int)std::min<int64_t>(S, cap / per_rung);
+    int Rc = (int)std::min<int64_t>(S, cap / per_rung);
+    if (pl->tune_rungs > 0) Rc = std::min(Rc, pl->tune_rungs);
     cplx<T>* Y = (cplx<T>*)d.Y;
     cplx<T>* Y2 = Y + cap;
     for (int r0 = 0; r0 < S; r0 += Rc) {
@@ -376,7 +377,8 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       }
       cplx<T>* Y1 = N == 3 && !slab ? Y2 : Y;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
-      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s)) {
+      if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s,
+                                 pl->tune_split)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
                                  twg, s);
@@ -533,6 +535,37 @@ static void graphs_clear(PlanGraphs& G) {
   G.key = GraphKey();
 }
 
+// Autotuned ladder launch shapes: the context table holds one entry per
+// (kind, N, B, S, precision, axis sizes); a plan looks it up again when the
+// table or its S changed, and drops its captured graphs if its shape moved.
+static std::vector<int64_t> tune_key(const pkde_plan* pl) {
+  std::vector<int64_t> k = {pl->kind, pl->g.N, pl->B, pl->S, pl->prec};
+  for (int i = 0; i < pl->g.N; ++i) k.push_back(pl->g.n[i]);
+  return k;
+}
+
+static void tune_apply(pkde_plan* pl) {
+  pkde_ctx* c = pl->ctx;
+  const int gen = c->tune_gen.load();
+  if (gen == pl->tune_gen && pl->S == pl->tune_S) return;
+  int rungs = 0, split = kAxisSplitTarget;
+  {
+    std::lock_guard<std::mutex> lk(c->tune_mu);
+    const std::vector<int64_t> k = tune_key(pl);
+    for (const auto& e : c->tune)
+      if (e.first == k) {
+        rungs = e.second.first;
+        split = e.second.second;
+        break;
+      }
+  }
+  if (rungs != pl->tune_rungs || split != pl->tune_split) graphs_clear(pl->graphs);
+  pl->tune_rungs = rungs;
+  pl->tune_split = split;
+  pl->tune_gen = gen;
+  pl->tune_S = pl->S;
+}
+
 // The estimate as three captured graphs, replayed while the caller repeats
 // the same buffers.  The first call with a new key runs directly; the second
 // captures; capture failures leave the plan on direct launches.  The timing
@@ -590,6 +623,7 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
   int rc = plan_reserve(pl, n, false);
   if (rc) return rc;
   if (pl->S_from_data && pl->kind == 0 && (rc = ladder_from_data(pl, d_x, n, s))) return rc;
+  tune_apply(pl);
   pl->last_n = n;
   if (pl->kind == 1)
     return pl->prec ? enqueue_rot<double>(pl, d_x, n, d_density, s, timed)
@@ -599,6 +633,96 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
                     : enqueue_graphs<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
   if (pl->prec) return enqueue_typed<double>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
   return enqueue_typed<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
+}
+
+// Ladder launch-shape autotuner.  Candidates: rungs per chunk, the even
+// splits of S into the fewest chunks the chunk buffer allows and up to three
+// more (more chunks: smaller tails per kernel, more spectra reads), times
+// the axis pass's workgroup target (no split, 4000, 8000, 16000).  Each is
+// timed on the plan's resident spectra (the ladder of its last estimate,
+// launched alone) `reps` times after one untimed run; the fastest goes to
+// the plan and the context table.  Per-rung arithmetic does not depend on
+// the launch shape, so results are bit-identical whatever is chosen.
+int autotune_plan(pkde_plan* pl, int reps, double* best_ms, double* default_ms) {
+  hipStream_t s = pl->ctx->stream;
+  PKDE_TRY(hipDeviceSynchronize(), "estimate");
+  std::vector<int> rungs_c = {0};
+  std::vector<int> split_c = {kAxisSplitTarget};
+  const GridDev& g = pl->g;
+  if (g.N >= 2 && pl->d.Y) {
+    const int64_t per_rung = 2 * (int64_t)pl->B * g.fsY;
+    const int64_t cap = pl->d.Y_elems / y_units(pl) / per_rung;
+    if (cap >= 1) {
+      const int S = pl->S;
+      const int c0 = (int)((S + cap - 1) / cap);
+      for (int c = c0 + 1; c <= c0 + 3 && c <= S; ++c) {
+        const int R = (S + c - 1) / c;
+        if (std::find(rungs_c.begin(), rungs_c.end(), R) == rungs_c.end()) rungs_c.push_back(R);
+      }
+    }
+    split_c = {kAxisSplitTarget, 0, 4000, 16000};
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PKDE_TRY(hipEventCreate(&e0), "event");
+  PKDE_TRY(hipEventCreate(&e1), "event");
+  const int keep_rungs = pl->tune_rungs, keep_split = pl->tune_split;
+  double best = 1e300, dflt = 0.0;
+  int best_r = 0, best_s = kAxisSplitTarget;
+  int rc = PKDE_OK;
+  for (int R : rungs_c) {
+    for (int sp : split_c) {
+      pl->tune_rungs = R;
+      pl->tune_split = sp;
+      float ms = 0.0f;
+      for (int it = 0; it <= reps && rc == PKDE_OK; ++it) {
+        if (it == 1 && hipEventRecord(e0, s) != hipSuccess) rc = PKDE_DEVICE_ERROR;
+        if (rc == PKDE_OK)
+          rc = pl->prec ? enqueue_typed<double>(pl, pl->d.x, pl->last_n, nullptr, nullptr, nullptr,
+                                                 s, false, 1)
+                        : enqueue_typed<float>(pl, pl->d.x, pl->last_n, nullptr, nullptr, nullptr,
+                                                s, false, 1);
+      }
+      if (rc == PKDE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = PKDE_DEVICE_ERROR;
+      if (rc != PKDE_OK) break;
+      const double t = (double)ms / reps;
+      if (R == 0 && sp == kAxisSplitTarget) dflt = t;
+      if (t < best) {
+        best = t;
+        best_r = R;
+        best_s = sp;
+      }
+    }
+    if (rc != PKDE_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PKDE_OK) {
+    pl->tune_rungs = keep_rungs;
+    pl->tune_split = keep_split;
+    return rc == PKDE_DEVICE_ERROR ? fail(rc, "autotune timing failed") : rc;
+  }
+  {
+    std::lock_guard<std::mutex> lk(pl->ctx->tune_mu);
+    const std::vector<int64_t> k = tune_key(pl);
+    bool found = false;
+    for (auto& e : pl->ctx->tune)
+      if (e.first == k) {
+        e.second = {best_r, best_s};
+        found = true;
+      }
+    if (!found) pl->ctx->tune.push_back({k, {best_r, best_s}});
+    pl->ctx->tune_gen.fetch_add(1);
+  }
+  graphs_clear(pl->graphs);
+  pl->tune_rungs = best_r;
+  pl->tune_split = best_s;
+  pl->tune_gen = pl->ctx->tune_gen.load();
+  pl->tune_S = pl->S;
+  if (best_ms) *best_ms = best;
+  if (default_ms) *default_ms = dflt;
+  return PKDE_OK;
 }
 
 }  // namespace pkde
@@ -1038,6 +1162,19 @@ int pkde_gradepro_graph_enable(pkde_plan* pl, int enable) {
   pl->graphs.failed = false;
   pl->graphs.last = pkde::GraphKey();
   return PKDE_OK;
+}
+
+int pkde_gradepro_autotune(pkde_plan* pl, int32_t reps, double* best_ms_out,
+                           double* default_ms_out) {
+  if (!pl) return fail(PKDE_INVALID_ARGUMENT, "null plan");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
+  if (pl->kind != 0) return fail(PKDE_INVALID_ARGUMENT, "autotune applies to :gradepro plans");
+  if (pl->last_n < 2 || !pl->d.vmr)
+    return fail(PKDE_INVALID_ARGUMENT, "run one estimate on the plan before tuning it");
+  if (reps < 1 || reps > 100) return fail(PKDE_INVALID_ARGUMENT, "reps must be 1..100");
+  std::lock_guard<std::mutex> lk(pl->ctx->mu);
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  return pkde::autotune_plan(pl, reps, best_ms_out, default_ms_out);
 }
 
 int pkde_gradepro_debug_enable(pkde_plan* pl, int enable) {

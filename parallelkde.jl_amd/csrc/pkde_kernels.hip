@@ -737,7 +737,8 @@ static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
-                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s) {
+                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s,
+                        int split_target) {
   AxisArgs<T> a{};
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[1];
@@ -788,11 +789,14 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.transposed = mode == 1;
   a.tw = (const cplx<T>*)tw;
   // With few tiles the last, partly filled round of blocks is a large share
-  // of the pass: split the chunk's rungs so that about 8000 blocks run,
-  // each keeping at least 8 rungs (it reads its spectra once per split).
-  // 256^2, 63-rung chunks: 3200 blocks -> 3 splits, ladder 12.13 -> 11.8 ms.
+  // of the pass: split the chunk's rungs so that about split_target blocks
+  // run, each keeping at least 8 rungs (it reads its spectra once per split).
+  // 256^2, 63-rung chunks, 8000: 3200 blocks -> 3 splits, ladder 12.13 -> 11.8 ms.
   const int64_t tiles = (int64_t)nf * (a.O / a.TJ) * a.ktiles;
-  a.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((8000 + tiles - 1) / tiles, R / 8));
+  a.nsplit = split_target <= 0 ? 1
+                               : (int)std::max<int64_t>(1, std::min<int64_t>(
+                                                               (split_target + tiles - 1) / tiles,
+                                                               R / 8));
   const int64_t blocks = tiles * a.nsplit;
   const size_t lds = (size_t)a.TK * a.TJ * pencil_stride(a.L) * sizeof(cplx<T>);
   return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
@@ -866,10 +870,10 @@ template bool launch_axis3_rows<float>(const GridDev&, const cplx<float>*, int64
                                        const float*, hipStream_t);
 template bool launch_axis_ladder<double>(const GridDev&, const cplx<double>*, cplx<double>*,
                                          int64_t, int, int, int, const double*, const double*,
-                                         int, const double*, int, hipStream_t);
+                                         int, const double*, int, hipStream_t, int);
 template bool launch_axis_ladder<float>(const GridDev&, const cplx<float>*, cplx<float>*,
                                         int64_t, int, int, int, const float*, const float*, int,
-                                        const float*, int, hipStream_t);
+                                        const float*, int, hipStream_t, int);
 
 template <typename T, int DIR>
 static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,

@@ -100,10 +100,14 @@ bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t s
                             const double* gV, const double* cV, const double* vscale,
                             const double* twg, const double* invk, double* vmr, hipStream_t s);
 
+// Workgroups the ladder axis pass aims at when a chunk has few tiles (its
+// rungs are then split over several workgroups per tile); 0: no split.
+constexpr int kAxisSplitTarget = 8000;
 template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
-                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s);
+                        const T* gV, int fsplit, const T* tw, int mode, hipStream_t s,
+                        int split_target = kAxisSplitTarget);
 
 // 3-D: inverse pass along grid axis 3 from the [k1][j2][j3] layout to the
 // tiled row-kernel layout.  False if n3 is outside the wave-local range.

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -39,6 +40,12 @@ struct pkde_ctx {
   // plans while holding `mu`.
   std::mutex live_mu;
   std::vector<pkde_plan*> live;
+  // autotuned ladder launch shapes, keyed by (kind, N, B, S, precision,
+  // axis sizes): (rungs per chunk, axis-pass workgroup target); tune_gen
+  // counts the entries so plans look the table up again only after a change
+  std::mutex tune_mu;
+  std::vector<std::pair<std::vector<int64_t>, std::pair<int, int>>> tune;
+  std::atomic<int> tune_gen{0};
 };
 
 namespace pkde {
@@ -135,6 +142,13 @@ struct pkde_plan {
   int64_t last_n = 0;
   int last_rc = PKDE_OK;
   pkde::PlanGraphs graphs;
+  // ladder launch shape (pkde_gradepro_autotune): rungs per chunk (0: as many
+  // as the chunk buffer holds) and the axis pass's workgroup target; the
+  // context table generation and S they were looked up for
+  int tune_rungs = 0;
+  int tune_split = pkde::kAxisSplitTarget;
+  int tune_gen = -1;
+  int tune_S = -1;
 };
 
 namespace pkde {

@@ -26,7 +26,7 @@ from ._lib import OK, INVALID_ARGUMENT, OUT_OF_GRID, NO_DEVICE, OUT_OF_MEMORY, U
 __all__ = [
     "ArgumentError", "DeviceUnavailableError", "Grid", "KDE", "DensityEstimation",
     "initialize_grid", "find_grid", "initialize_kde", "initialize_estimation",
-    "estimate_density", "estimate_density_", "get_density", "has_grid", "get_grid",
+    "estimate_density", "estimate_density_", "get_density", "has_grid", "get_grid", "autotune",
     "spacings", "bounds", "low_bounds", "high_bounds", "initial_bandwidth", "fftgrid",
     "AVAILABLE_DEVICES", "DEVICE_IMPLEMENTATIONS", "DEFAULT_IMPLEMENTATIONS",
     "estimator_lookup", "GradeProResult", "RoTResult", "estimate_batch", "Server", "context",
@@ -451,6 +451,15 @@ class _Plan:
                               np.array(st.bw_final[: self.grid.ndims]), st.ms_total, st.ms_ladder,
                               st.algorithmic_bytes)
 
+    def autotune(self, reps=3):
+        """Ladder launch-shape autotuner (pkde_gradepro_autotune): returns the
+        ladder milliseconds of the chosen shape and of the default one."""
+        best = _lib.C.c_double()
+        default = _lib.C.c_double()
+        _raise(_lib.load().pkde_gradepro_autotune(self.handle, int(reps), _lib.C.byref(best),
+                                                  _lib.C.byref(default)), "autotune")
+        return best.value, default.value
+
     def debug_vmr(self, n_rungs):
         G = int(np.prod(self.grid.shape))
         out = np.empty(n_rungs * G)
@@ -596,6 +605,22 @@ def estimate_density(est: DensityEstimation, estimation_method="gradepro", metho
 
 
 estimate_density_ = estimate_density
+
+
+def autotune(est: DensityEstimation, reps=3):
+    """Tune the ladder launch shape of every :gradepro plan the estimation
+    has run (run estimate_density first).  The choice is kept for plans of the
+    same grid, B, ladder length and precision in this process's device
+    context, the batched and serving forms included; densities are unchanged.
+    Returns [(ladder ms chosen, ladder ms default), ...].  No counterpart in
+    the reference."""
+    out = []
+    for plan in est._plans.values():
+        if isinstance(plan, _Plan):
+            out.append(plan.autotune(reps))
+    if not out:
+        raise ArgumentError("run estimate_density(est, :gradepro) before autotune")
+    return out
 
 
 # ------------------------------------------------------------ batched form

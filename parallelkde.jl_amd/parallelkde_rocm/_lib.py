@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = [
     "pkde_gradepro_estimate", "pkde_gradepro_estimate_device", "pkde_gradepro_check",
     "pkde_gradepro_last_stats",
     "pkde_gradepro_debug_enable", "pkde_gradepro_debug_vmr", "pkde_gradepro_convergence",
-    "pkde_gradepro_graph_enable",
+    "pkde_gradepro_graph_enable", "pkde_gradepro_autotune",
     "pkde_dirac_sequences", "pkde_gradepro_estimate_batch",
     "pkde_server_create", "pkde_server_submit", "pkde_server_poll", "pkde_server_wait",
     "pkde_server_stats_get", "pkde_server_destroy",
@@ -126,6 +126,7 @@ def load() -> C.CDLL:
     L.pkde_gradepro_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.pkde_gradepro_debug_enable.argtypes = [vp, C.c_int]
     L.pkde_gradepro_graph_enable.argtypes = [vp, C.c_int]
+    L.pkde_gradepro_autotune.argtypes = [vp, C.c_int32, vp, vp]
     L.pkde_gradepro_debug_vmr.argtypes = [vp, vp, C.c_int64]
     L.pkde_gradepro_convergence.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, C.c_double, C.c_int32,
                                             C.c_int32, vp]

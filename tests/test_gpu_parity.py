@@ -538,6 +538,39 @@ def test_graph_replay_matches_direct(pk, L, shape):
             hip.hipFree(p)
 
 
+@pytest.mark.parametrize("shape", [[(-6.0, 6.0, 64)] * 2, [(-4.0, 4.0, 32)] * 3,
+                                   [(-6.0, 6.0, 48)] * 2, [(-5.0, 5.0, 512)]],
+                         ids=["2d-64", "3d-32", "2d-48", "1d-512"])
+def test_autotune_keeps_results(pk, shape):
+    """The ladder launch-shape autotuner (pkde_gradepro_autotune) chooses
+    among shapes with the same per-rung arithmetic: the log-VMR table its last
+    candidate wrote, the estimate after tuning and the estimate of a new plan
+    of the same shape (which takes the tuned shape from the context) all equal
+    the untuned results bit for bit.  Tuning before any estimate is an error."""
+    nd = len(shape)
+    x = np.ascontiguousarray({1: lambda: W.gaussian_1d(10000, 46)[:, None],
+                              2: lambda: W.mixture_2d(20000, 47),
+                              3: lambda: W.correlated_3d(20000, 48)}[nd]().reshape(-1, nd))
+    grid = pk.initialize_grid(shape)
+    params = pk._params(nd, n_bootstraps=20, n_steps=60)
+    plan = pk._Plan(grid, params)
+    with pytest.raises(pk.ArgumentError):
+        plan.autotune()
+    L = pk._lib.load()
+    assert L.pkde_gradepro_debug_enable(plan.handle, 1) == 0
+    r0 = plan.estimate(x)
+    v0 = plan.debug_vmr(61)
+    best, default = plan.autotune(reps=2)
+    assert 0.0 < best <= default
+    assert np.array_equal(plan.debug_vmr(61), v0, equal_nan=True)
+    r1 = plan.estimate(x)
+    assert np.array_equal(r1.density, r0.density) and np.array_equal(r1.chosen_rung, r0.chosen_rung)
+    fresh = pk._Plan(grid, params)
+    r2 = fresh.estimate(x)
+    assert np.array_equal(r2.density, r0.density) and np.array_equal(r2.chosen_rung, r0.chosen_rung)
+    say(f"  autotuned ladder {best:.3f} ms (default shape {default:.3f} ms); results unchanged")
+
+
 def test_deterministic_repeat(pk):
     """Fixed-point binning + fixed reduction orders: bitwise repeatable."""
     x = W.mixture_2d(30000, 9)
