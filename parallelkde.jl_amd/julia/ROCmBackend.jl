@@ -178,7 +178,12 @@ function DensityEstimators.initialize_estimator(
 end
 
 # One ccall per estimate (GradeProEstimator.jl:968-1072 on the device).
-function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N}; method::Symbol, kwargs...) where {N}
+# autotune=true (no counterpart in the reference): after the estimate, time the
+# ladder's launch shapes for this grid on the device (pkde_gradepro_autotune);
+# later estimates of the same grid, bootstraps, ladder and precision use the
+# fastest.  The density does not change.
+function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N}; method::Symbol,
+                                     autotune::Bool=false, kwargs...) where {N}
     n = size(kde.data, 2)
     chosen = Array{Int16}(undef, size(kde.density))
     est.plan == C_NULL && throw(ArgumentError("estimator plan was destroyed"))
@@ -186,6 +191,10 @@ function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N
         check(ccall((:pkde_gradepro_estimate, libpkde), Cint,
             (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Int16}, Ptr{Cvoid}),
             est.plan, kde.data, n, C_NULL, kde.density, chosen, C_NULL))
+    end
+    if autotune
+        check(ccall((:pkde_gradepro_autotune, libpkde), Cint,
+            (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}), est.plan, Int32(3), C_NULL, C_NULL))
     end
     return nothing
 end
