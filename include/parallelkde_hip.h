@@ -171,7 +171,10 @@ int pkde_gradepro_graph_enable(pkde_plan *plan, int enable);
  * resident; run one estimate first), `reps` times after one untimed run, for
  * each candidate: rungs per chunk (the even splits of the ladder into the
  * fewest chunks the chunk buffer allows and up to three more) times the axis
- * pass's workgroup target (no split, 4000, 8000, 16000).  The fastest is
+ * pass's workgroup target (no split, 4000, 8000, 16000); then, on 2-D and
+ * 3-D grids whose last estimate took its samples from the plan's own buffer
+ * (pkde_gradepro_estimate), the binning's deposit tile (8192, 4096 or 2048
+ * cells), timed with the binning and forward transforms.  The fastest is
  * kept for this plan and recorded in the context, where every plan of the
  * same shape (the batched and serving forms included) picks it up.  Later
  * estimates are bit-identical to untuned ones: the per-rung arithmetic does

@@ -314,7 +314,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
                      (T*)d.gV, s);
     if (sorted) {
       if (launch_binning_sorted(g, d_x, n, d_idx, B, pl->p.rng_seed, frac_bits_for(n), d.acc, d.err,
-                                d.binwork, d.binwork_bytes, s))
+                                d.binwork, d.binwork_bytes, s, pl->tune.tile))
         return fail(PKDE_DEVICE_ERROR, "binning workspace does not fit the estimate");
     } else {
       launch_binning(g, d_x, n, d_idx, B, pl->p.rng_seed, pl->fixed, frac_bits_for(n), d.acc, d.err,
@@ -355,7 +355,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
     const int64_t cap = d.Y_elems / y_units(pl);
     if (cap < per_rung) return fail(PKDE_DEVICE_ERROR, "rung chunk buffer smaller than one rung");
     int Rc = (int)std::min<int64_t>(S, cap / per_rung);
-    if (pl->tune_rungs > 0) Rc = std::min(Rc, pl->tune_rungs);
+    if (pl->tune.rungs > 0) Rc = std::min(Rc, pl->tune.rungs);
     cplx<T>* Y = (cplx<T>*)d.Y;
     cplx<T>* Y2 = Y + cap;
     for (int r0 = 0; r0 < S; r0 += Rc) {
@@ -378,7 +378,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       cplx<T>* Y1 = N == 3 && !slab ? Y2 : Y;
       const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s,
-                                 pl->tune_split)) {
+                                 pl->tune.split)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
                                  twg, s);
@@ -548,20 +548,18 @@ static void tune_apply(pkde_plan* pl) {
   pkde_ctx* c = pl->ctx;
   const int gen = c->tune_gen.load();
   if (gen == pl->tune_gen && pl->S == pl->tune_S) return;
-  int rungs = 0, split = kAxisSplitTarget;
+  TuneShape t;
   {
     std::lock_guard<std::mutex> lk(c->tune_mu);
     const std::vector<int64_t> k = tune_key(pl);
     for (const auto& e : c->tune)
       if (e.first == k) {
-        rungs = e.second.first;
-        split = e.second.second;
+        t = e.second;
         break;
       }
   }
-  if (rungs != pl->tune_rungs || split != pl->tune_split) graphs_clear(pl->graphs);
-  pl->tune_rungs = rungs;
-  pl->tune_split = split;
+  if (!(t == pl->tune)) graphs_clear(pl->graphs);
+  pl->tune = t;
   pl->tune_gen = gen;
   pl->tune_S = pl->S;
 }
@@ -624,6 +622,8 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
   if (rc) return rc;
   if (pl->S_from_data && pl->kind == 0 && (rc = ladder_from_data(pl, d_x, n, s))) return rc;
   tune_apply(pl);
+  pl->last_x = d_x;
+  pl->last_idx = d_idx;
   pl->last_n = n;
   if (pl->kind == 1)
     return pl->prec ? enqueue_rot<double>(pl, d_x, n, d_density, s, timed)
@@ -635,23 +635,29 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
   return enqueue_typed<float>(pl, d_x, n, d_idx, d_density, d_chosen, s, timed);
 }
 
-// Ladder launch-shape autotuner.  Candidates: rungs per chunk, the even
+// Launch-shape autotuner.  Ladder candidates: rungs per chunk, the even
 // splits of S into the fewest chunks the chunk buffer allows and up to three
-// more (more chunks: smaller tails per kernel, more spectra reads), times
-// the axis pass's workgroup target (no split, 4000, 8000, 16000).  Each is
-// timed on the plan's resident spectra (the ladder of its last estimate,
-// launched alone) `reps` times after one untimed run; the fastest goes to
-// the plan and the context table.  Per-rung arithmetic does not depend on
-// the launch shape, so results are bit-identical whatever is chosen.
+// more (more chunks: smaller tails per kernel, more spectra reads), times the
+// axis pass's workgroup target (no split, 4000, 8000, 16000); each is timed
+// on the plan's resident spectra (the ladder of its last estimate, launched
+// alone).  Binning candidates (sorted fixed-point binning, when the last
+// estimate's samples and draws are in the plan's own buffers): 8192, 4096 or
+// 2048 cells per deposit tile (smaller tiles: more workgroups per CU and
+// less imbalance between crowded and empty tiles, more samples read twice at
+// tile edges), timed with the binning and forward transforms of that
+// estimate.  Each candidate runs once untimed, then `reps` times; the
+// fastest goes to the plan and the context table.  Integer deposits and the
+// per-rung arithmetic do not depend on the launch shape, so results are
+// bit-identical whatever is chosen.
 int autotune_plan(pkde_plan* pl, int reps, double* best_ms, double* default_ms) {
   hipStream_t s = pl->ctx->stream;
   PKDE_TRY(hipDeviceSynchronize(), "estimate");
-  std::vector<int> rungs_c = {0};
-  std::vector<int> split_c = {kAxisSplitTarget};
   const GridDev& g = pl->g;
+  std::vector<TuneShape> ladder_c(1);
   if (g.N >= 2 && pl->d.Y) {
     const int64_t per_rung = 2 * (int64_t)pl->B * g.fsY;
     const int64_t cap = pl->d.Y_elems / y_units(pl) / per_rung;
+    std::vector<int> rungs_c = {0};
     if (cap >= 1) {
       const int S = pl->S;
       const int c0 = (int)((S + cap - 1) / cap);
@@ -660,47 +666,77 @@ int autotune_plan(pkde_plan* pl, int reps, double* best_ms, double* default_ms) 
         if (std::find(rungs_c.begin(), rungs_c.end(), R) == rungs_c.end()) rungs_c.push_back(R);
       }
     }
-    split_c = {kAxisSplitTarget, 0, 4000, 16000};
+    ladder_c.clear();
+    for (int R : rungs_c)
+      for (int sp : {kAxisSplitTarget, 0, 4000, 16000}) {
+        TuneShape t;
+        t.rungs = R;
+        t.split = sp;
+        ladder_c.push_back(t);
+      }
   }
+  // (1-D tiles are one grid row whatever the capacity)
+  const bool bin_tunable = g.N >= 2 && pl->fixed && pl->d.binwork && pl->last_x == pl->d.x &&
+                           (pl->last_idx == nullptr || pl->last_idx == pl->d.idx);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PKDE_TRY(hipEventCreate(&e0), "event");
   PKDE_TRY(hipEventCreate(&e1), "event");
-  const int keep_rungs = pl->tune_rungs, keep_split = pl->tune_split;
-  double best = 1e300, dflt = 0.0;
-  int best_r = 0, best_s = kAxisSplitTarget;
+  const TuneShape keep = pl->tune;
+  // mean time of `reps` runs of segment seg (0: binning + forward
+  // transforms, 1: ladder) with shape t, after one untimed run
+  auto time_seg = [&](const TuneShape& t, int seg, double& out) -> int {
+    pl->tune = t;
+    for (int it = 0; it <= reps; ++it) {
+      if (it == 1 && hipEventRecord(e0, s) != hipSuccess) return PKDE_DEVICE_ERROR;
+      const int rc = pl->prec ? enqueue_typed<double>(pl, pl->last_x, pl->last_n, pl->last_idx,
+                                                      nullptr, nullptr, s, false, seg)
+                              : enqueue_typed<float>(pl, pl->last_x, pl->last_n, pl->last_idx,
+                                                     nullptr, nullptr, s, false, seg);
+      if (rc != PKDE_OK) return rc;
+    }
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return PKDE_DEVICE_ERROR;
+    out = (double)ms / reps;
+    return PKDE_OK;
+  };
+  TuneShape best = keep;
+  best.rungs = 0;
+  best.split = kAxisSplitTarget;
+  double best_t = 1e300, dflt = 0.0;
   int rc = PKDE_OK;
-  for (int R : rungs_c) {
-    for (int sp : split_c) {
-      pl->tune_rungs = R;
-      pl->tune_split = sp;
-      float ms = 0.0f;
-      for (int it = 0; it <= reps && rc == PKDE_OK; ++it) {
-        if (it == 1 && hipEventRecord(e0, s) != hipSuccess) rc = PKDE_DEVICE_ERROR;
-        if (rc == PKDE_OK)
-          rc = pl->prec ? enqueue_typed<double>(pl, pl->d.x, pl->last_n, nullptr, nullptr, nullptr,
-                                                 s, false, 1)
-                        : enqueue_typed<float>(pl, pl->d.x, pl->last_n, nullptr, nullptr, nullptr,
-                                                s, false, 1);
-      }
-      if (rc == PKDE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        rc = PKDE_DEVICE_ERROR;
-      if (rc != PKDE_OK) break;
-      const double t = (double)ms / reps;
-      if (R == 0 && sp == kAxisSplitTarget) dflt = t;
-      if (t < best) {
-        best = t;
-        best_r = R;
-        best_s = sp;
+  for (const TuneShape& c : ladder_c) {
+    TuneShape t = c;
+    t.tile = keep.tile;
+    double ms = 0.0;
+    if ((rc = time_seg(t, 1, ms)) != PKDE_OK) break;
+    if (t.rungs == 0 && t.split == kAxisSplitTarget) dflt = ms;
+    if (ms < best_t) {
+      best_t = ms;
+      best.rungs = t.rungs;
+      best.split = t.split;
+    }
+  }
+  // the binning runs recompute the resident spectra and tables bit for bit
+  if (rc == PKDE_OK && bin_tunable) {
+    double tbest = 1e300;
+    for (int tile : {kBinTileCells, kBinTileCells / 2, kBinTileCells / 4}) {
+      if (g.n[0] > tile) continue;
+      TuneShape t = best;
+      t.tile = tile;
+      double ms = 0.0;
+      if ((rc = time_seg(t, 0, ms)) != PKDE_OK) break;
+      if (ms < tbest) {
+        tbest = ms;
+        best.tile = tile;
       }
     }
-    if (rc != PKDE_OK) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != PKDE_OK) {
-    pl->tune_rungs = keep_rungs;
-    pl->tune_split = keep_split;
+    pl->tune = keep;
     return rc == PKDE_DEVICE_ERROR ? fail(rc, "autotune timing failed") : rc;
   }
   {
@@ -709,18 +745,17 @@ int autotune_plan(pkde_plan* pl, int reps, double* best_ms, double* default_ms) 
     bool found = false;
     for (auto& e : pl->ctx->tune)
       if (e.first == k) {
-        e.second = {best_r, best_s};
+        e.second = best;
         found = true;
       }
-    if (!found) pl->ctx->tune.push_back({k, {best_r, best_s}});
+    if (!found) pl->ctx->tune.push_back({k, best});
     pl->ctx->tune_gen.fetch_add(1);
   }
   graphs_clear(pl->graphs);
-  pl->tune_rungs = best_r;
-  pl->tune_split = best_s;
+  pl->tune = best;
   pl->tune_gen = pl->ctx->tune_gen.load();
   pl->tune_S = pl->S;
-  if (best_ms) *best_ms = best;
+  if (best_ms) *best_ms = best_t;
   if (default_ms) *default_ms = dflt;
   return PKDE_OK;
 }

@@ -27,7 +27,7 @@ namespace pkde {
 namespace {
 
 constexpr int kBinThreads = 1024;
-constexpr int kTileCells = 8192;           // 2 x 8192 int64 = 128 KB of LDS
+constexpr int kTileCells = kBinTileCells;  // at most: 2 x 8192 int64 = 128 KB of LDS
 constexpr int kCountChunk = 131072;        // samples per counting block (uint8)
 constexpr int kScanBlock = 4096;           // elements per scan block (1024 x 4)
 
@@ -389,12 +389,12 @@ __global__ __launch_bounds__(kThreads) void k_bin_fallback(GridDev g, const doub
   }
 }
 
-// Tiles span axis 1 and hold at most kTileCells cells (n1 <= 4096).
-TileGeom tile_geom(const GridDev& g) {
+// Tiles span axis 1 and hold at most `cells` cells (n1 <= cells).
+TileGeom tile_geom(const GridDev& g, int cells) {
   TileGeom t{};
   const int n1 = g.n[0];
   const int n2 = g.N >= 2 ? g.n[1] : 1, n3 = g.N == 3 ? g.n[2] : 1;
-  const int rows = std::max(1, kTileCells / n1);  // rows of n1 cells per tile
+  const int rows = std::max(1, cells / n1);  // rows of n1 cells per tile
   if (g.N == 1) {
     t.ty = 1;
     t.tz = 1;
@@ -454,9 +454,10 @@ uint64_t binsort_bytes(const GridDev& g, int64_t n, int B) {
 
 int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
                           uint64_t seed, int frac_bits, void* acc, int* err, void* work,
-                          uint64_t work_bytes, hipStream_t s) {
+                          uint64_t work_bytes, hipStream_t s, int tile_cells) {
   if (g.N > 3 || binsort_bytes(g, n, B) > work_bytes) return -1;
-  if (g.G + 2 >= (1ll << 31) || n >= (1ll << 31) || g.n[0] > kTileCells) return -1;
+  if (tile_cells <= 0 || tile_cells > kTileCells) tile_cells = kTileCells;
+  if (g.G + 2 >= (1ll << 31) || n >= (1ll << 31) || g.n[0] > tile_cells) return -1;
   const int64_t npad = (n + 3) & ~(int64_t)3;
   unsigned char* p = (unsigned char*)work;
   auto take = [&](uint64_t v) {
@@ -511,7 +512,7 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
                                                                    key1, cnt, ovf, err);
   }
   // 3. tile deposits for every bootstrap and the full sample
-  const TileGeom tg = tile_geom(g);
+  const TileGeom tg = tile_geom(g, tile_cells);
   const unsigned nblk = (unsigned)((B + 1) * tg.tiles_y * tg.tiles_z);
   const size_t lds = (size_t)2 * tg.T * sizeof(unsigned long long);
   auto deposit = g.N == 1 ? k_bin_deposit<1> : g.N == 2 ? k_bin_deposit<2> : k_bin_deposit<3>;

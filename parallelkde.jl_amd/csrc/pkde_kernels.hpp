@@ -55,9 +55,14 @@ void launch_binning(const GridDev& g, const double* x, int64_t n, const int32_t*
 // launch_binning with fixed != 0.  `work` must hold binsort_bytes(); returns
 // nonzero (nothing launched) if it does not or the sizes are out of range.
 uint64_t binsort_bytes(const GridDev& g, int64_t n, int B);
+// Grid cells per tile of the sorted binning's deposits (2 int64 fields in
+// LDS: 128 KB); tiles span axis 1, so n1 must not exceed it.
+constexpr int kBinTileCells = 8192;
+
 int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const int32_t* idx, int B,
                           uint64_t seed, int frac_bits, void* acc, int* err, void* work,
-                          uint64_t work_bytes, hipStream_t s);
+                          uint64_t work_bytes, hipStream_t s,
+                          int tile_cells = kBinTileCells);
 
 // acc -> forward transform along axis 1 in FP64, Hermitian halves written to
 // specD[fid][k1][q].

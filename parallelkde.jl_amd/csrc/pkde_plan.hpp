@@ -15,6 +15,20 @@ struct pkde_plan;
 
 // Stream lane of the batched form: its own stream and the plans (device
 // buffers) it keeps resident between calls, keyed by grid and parameters.
+namespace pkde {
+// Launch shape chosen by the autotuner: ladder rungs per chunk (0: as many
+// as the chunk buffer holds), the ladder axis pass's workgroup target, and
+// the cells per tile of the sorted binning.
+struct TuneShape {
+  int rungs = 0;
+  int split = kAxisSplitTarget;
+  int tile = kBinTileCells;
+  bool operator==(const TuneShape& o) const {
+    return rungs == o.rungs && split == o.split && tile == o.tile;
+  }
+};
+}  // namespace pkde
+
 struct pkde_batch_lane {
   hipStream_t stream = nullptr;
   std::vector<std::pair<std::vector<unsigned char>, pkde_plan*>> plans;
@@ -40,11 +54,11 @@ struct pkde_ctx {
   // plans while holding `mu`.
   std::mutex live_mu;
   std::vector<pkde_plan*> live;
-  // autotuned ladder launch shapes, keyed by (kind, N, B, S, precision,
-  // axis sizes): (rungs per chunk, axis-pass workgroup target); tune_gen
-  // counts the entries so plans look the table up again only after a change
+  // autotuned launch shapes, keyed by (kind, N, B, S, precision, axis
+  // sizes); tune_gen counts the entries so plans look the table up again
+  // only after a change
   std::mutex tune_mu;
-  std::vector<std::pair<std::vector<int64_t>, std::pair<int, int>>> tune;
+  std::vector<std::pair<std::vector<int64_t>, pkde::TuneShape>> tune;
   std::atomic<int> tune_gen{0};
 };
 
@@ -140,13 +154,13 @@ struct pkde_plan {
   pkde::PlanBuffers d;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t last_n = 0;
+  const double* last_x = nullptr;     // samples and draws of the last estimate
+  const int32_t* last_idx = nullptr;
   int last_rc = PKDE_OK;
   pkde::PlanGraphs graphs;
-  // ladder launch shape (pkde_gradepro_autotune): rungs per chunk (0: as many
-  // as the chunk buffer holds) and the axis pass's workgroup target; the
-  // context table generation and S they were looked up for
-  int tune_rungs = 0;
-  int tune_split = pkde::kAxisSplitTarget;
+  // launch shape (pkde_gradepro_autotune); the context table generation and
+  // S it was looked up for
+  pkde::TuneShape tune;
   int tune_gen = -1;
   int tune_S = -1;
 };

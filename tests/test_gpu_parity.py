@@ -542,11 +542,12 @@ def test_graph_replay_matches_direct(pk, L, shape):
                                    [(-6.0, 6.0, 48)] * 2, [(-5.0, 5.0, 512)]],
                          ids=["2d-64", "3d-32", "2d-48", "1d-512"])
 def test_autotune_keeps_results(pk, shape):
-    """The ladder launch-shape autotuner (pkde_gradepro_autotune) chooses
-    among shapes with the same per-rung arithmetic: the log-VMR table its last
-    candidate wrote, the estimate after tuning and the estimate of a new plan
-    of the same shape (which takes the tuned shape from the context) all equal
-    the untuned results bit for bit.  Tuning before any estimate is an error."""
+    """The launch-shape autotuner (pkde_gradepro_autotune) chooses among
+    ladder shapes with the same per-rung arithmetic and binning tiles with the
+    same integer deposits: the log-VMR tables its candidates wrote, the
+    estimate after tuning and the estimate of a new plan of the same shape
+    (which takes the tuned shape from the context) all equal the untuned
+    results bit for bit.  Tuning before any estimate is an error."""
     nd = len(shape)
     x = np.ascontiguousarray({1: lambda: W.gaussian_1d(10000, 46)[:, None],
                               2: lambda: W.mixture_2d(20000, 47),
@@ -562,6 +563,11 @@ def test_autotune_keeps_results(pk, shape):
     v0 = plan.debug_vmr(61)
     best, default = plan.autotune(reps=2)
     assert 0.0 < best <= default
+    assert np.array_equal(plan.debug_vmr(61), v0, equal_nan=True)
+    # the tuner's last binning candidate (2048-cell deposit tiles on 2-D and
+    # 3-D grids) left the resident spectra; a second tuning runs its ladders
+    # on them, so the table now reflects that tile size
+    plan.autotune(reps=1)
     assert np.array_equal(plan.debug_vmr(61), v0, equal_nan=True)
     r1 = plan.estimate(x)
     assert np.array_equal(r1.density, r0.density) and np.array_equal(r1.chosen_rung, r0.chosen_rung)
