@@ -804,7 +804,9 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
   while (hp < a.H1) hp <<= 1;
-  a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(8, per_block)) : 1;
+  // mode 2: TJ pencils along j3 give TJ-value contiguous runs of the output;
+  // 16 (256-byte runs) measured 0.7 % faster than 8 at 64^3 and 128^3, 32 slower
+  a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(16, per_block)) : 1;
   a.TK = std::min(hp, per_block / a.TJ);
   if (mode == 1 && a.TK != g.TKy) return false;  // tile width fixed by the layout
   a.ktiles = (a.H1 + a.TK - 1) / a.TK;
