@@ -187,7 +187,9 @@ int pkde_gradepro_autotune(pkde_plan *plan, int32_t reps, double *best_ms_out,
 
 /* Per-rung log-VMR table of the last estimate (S x G, rung-major), for
  * primitive-level parity (calculate_scaled_vmr!, DirectSpace.jl:526-584).
- * Only available when the plan was created with debug tables enabled. */
+ * Copies the first min(capacity / G, S) whole rungs (capacity in doubles;
+ * less than one rung is PKDE_INVALID_ARGUMENT).  debug_enable is kept for
+ * ABI compatibility: the table of the last estimate is always resident. */
 int pkde_gradepro_debug_enable(pkde_plan *plan, int enable);
 int pkde_gradepro_debug_vmr(pkde_plan *plan, double *vmr_out, int64_t capacity);
 
@@ -208,9 +210,14 @@ int pkde_dirac_sequences(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
 
 /* ---- batched form: many independent estimates in one submission ----
  * Requests are spread over up to PKDE_BATCH_STREAMS (default 8) device
- * streams and run concurrently; device buffers stay resident between calls.
- * Results are written when the call returns.  No counterpart in the
- * reference, which estimates one dataset per call. */
+ * streams and run concurrently.  Each stream keeps one plan per grid shape
+ * and keywords (the seed is passed per request) and one rung-chunk buffer
+ * that its plans share, resident between calls; when the plans a call needs
+ * do not fit the free memory, the streams first drop the plans it does not
+ * use.  A request that fails (invalid, out of grid, out of memory) gets its
+ * own status in status_out; the others complete.  Results are written when
+ * the call returns.  No counterpart in the reference, which estimates one
+ * dataset per call. */
 typedef struct {
   int32_t ndims;
   int64_t n_per_axis[PKDE_MAX_DIMS];
@@ -224,11 +231,26 @@ typedef struct {
 
 int pkde_gradepro_estimate_batch(pkde_ctx *ctx, int32_t count, const pkde_request *reqs,
                                  int32_t *status_out);
+/* Frees what the batched form keeps resident in the context (its streams'
+ * plans and chunk buffers); the next batch allocates them again. */
+int pkde_context_release_batch(pkde_ctx *ctx);
+/* Device bytes a request needs (its plan for req->n samples and the chunk
+ * buffer its stream lends the plan), computed on the host without any
+ * device call: the admission figure of the batched and serving forms
+ * (extends get_necessary_memory, GradeProEstimator.jl:1091-1096). */
+int pkde_gradepro_request_footprint(pkde_ctx *ctx, const pkde_request *req, uint64_t *bytes_out);
 
 /* ---- serving form: continuous batching with HBM-footprint admission ----
- * A server thread admits requests while their footprint fits the budget and
- * a stream slot is free; requests that arrive while others run overlap on the
- * device.  Results do not depend on what else is in flight. */
+ * A server thread admits requests, first come first served, as soon as a
+ * stream slot is free and the device memory the request still needs (a plan
+ * for its shape, a larger chunk buffer for the slot) fits the budget next to
+ * everything resident (bytes_in_use: the pooled plans and the slots' chunk
+ * buffers); idle plans are evicted, least recently used first, to make room.
+ * A request whose footprint alone exceeds the budget completes with
+ * PKDE_OUT_OF_MEMORY; the others go on.  Plans are pooled per grid shape and
+ * keywords and reused by later requests; none is freed while a stream uses
+ * it.  Requests that arrive while others run overlap on the device.  Results
+ * do not depend on what else is in flight. */
 typedef struct {
   int32_t max_inflight;      /* concurrent requests (device streams) */
   uint64_t memory_budget;    /* bytes of device memory for admitted requests */

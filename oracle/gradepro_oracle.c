@@ -207,9 +207,22 @@ int og_check_data(const og_grid *g, const double *samples, int64_t n) {
   return OG_OK;
 }
 
-/* generate_dirac_cpu! (DirectSpace.jl:196-242) for one bootstrap column. */
-static int og_deposit_reversed = 0;
-void og_set_deposit_order(int reversed) { og_deposit_reversed = reversed ? 1 : 0; }
+/* generate_dirac_cpu! (DirectSpace.jl:196-242) for one bootstrap column.
+ * Deposit modes (og_set_deposit_order): 0 the reference's FP64 sums in draw
+ * order; 1 the same sums last draw first (a rounding-envelope probe); 2 the
+ * optional fixed-point deterministic mode: each deposit is rounded to
+ * llrint(term * 2^fb * n * prod h) (its square to llrint(term^2 * 2^fb *
+ * (n * prod h)^2)) and summed exactly in 64-bit integers, fb = 61 - (bits of
+ * n), then scaled back as (double)sum * (1 / scale) -- the same rounding of
+ * the same sum, in a form whose result does not depend on the order. */
+static int og_deposit_mode = 0;
+void og_set_deposit_order(int mode) { og_deposit_mode = (mode == 1 || mode == 2) ? mode : 0; }
+
+int og_fixed_frac_bits(int64_t n) {
+  int lg = 0;
+  while ((1ll << lg) <= n) ++lg;
+  return 61 - lg;
+}
 
 static void og_dirac_one(const og_grid *g, const double *samples, int64_t n,
                          const int32_t *col, double *sums, double *squares) {
@@ -222,8 +235,13 @@ static void og_dirac_one(const og_grid *g, const double *samples, int64_t n,
   stride[0] = 1;
   for (int i = 1; i < N; ++i) stride[i] = stride[i - 1] * g->n[i - 1];
 
+  const int fixed = og_deposit_mode == 2;
+  const double two_fb = ldexp(1.0, og_fixed_frac_bits(n));
+  const double scaleM = (double)n * hp * two_fb;
+  const double scaleV = ((double)n * hp) * ((double)n * hp) * two_fb;
+  int64_t *isums = (int64_t *)sums, *isq = (int64_t *)squares;
   for (int64_t s0 = 0; s0 < n; ++s0) {
-    const int64_t s = og_deposit_reversed ? n - 1 - s0 : s0;
+    const int64_t s = og_deposit_mode == 1 ? n - 1 - s0 : s0;
     const double *x = samples + (int64_t)col[s] * N;
     int64_t il[OG_MAX_DIMS];
     double rl[OG_MAX_DIMS], rh[OG_MAX_DIMS];
@@ -250,9 +268,21 @@ static void og_dirac_one(const og_grid *g, const double *samples, int64_t n,
         }
       }
       double term = (pl * ph) / denom;
+      if (fixed) {
+        isums[off] += (int64_t)llrint(term * scaleM);
+        if (squares) isq[off] += (int64_t)llrint((term * term) * scaleV);
+        continue;
+      }
       sums[off] += term;
       if (squares) squares[off] += term * term;
     }
+  }
+  if (fixed) {
+    const int64_t G = og_grid_points(g);
+    const double iM = 1.0 / scaleM, iV = 1.0 / scaleV;
+    for (int64_t j = 0; j < G; ++j) sums[j] = (double)isums[j] * iM;
+    if (squares)
+      for (int64_t j = 0; j < G; ++j) squares[j] = (double)isq[j] * iV;
   }
 }
 
@@ -283,6 +313,14 @@ typedef struct {
 
 static int og_fft_mode = 0;
 void og_set_fft_order(int mode) { og_fft_mode = (mode == 1 || mode == 2) ? mode : 0; }
+
+/* 1: the ladder keeps only the real part of each inverse transform (the
+ * exact result is real: a complex-to-real inverse, as a Hermitian-packed
+ * implementation computes it); 0: the reference's |.| of the complex
+ * result, whose imaginary part is rounding noise.  A rounding-envelope
+ * probe; not thread-safe. */
+static int og_real_output = 0;
+void og_set_real_output(int on) { og_real_output = on ? 1 : 0; }
 
 static int og_fft1_init(og_fft1 *p, int64_t n) {
   p->n = n;
@@ -696,6 +734,11 @@ int og_gradepro_estimate(const og_grid *g, const double *samples, int64_t n,
         mt[j] *= ginv;
         vt[j] *= ginv;
       }
+      if (og_real_output)
+        for (int64_t j = 0; j < G; ++j) {
+          mt[2 * j + 1] = 0.0;
+          vt[2 * j + 1] = 0.0;
+        }
     }
     /* VMR over points; split the point range across threads */
     {
@@ -711,6 +754,8 @@ int og_gradepro_estimate(const og_grid *g, const double *samples, int64_t n,
     og_propagate(g, kgrid, tk, t0, F0, NULL, Ft, NULL);
     og_fft_nd(N, g->n, Ft, +1);
     for (int64_t j = 0; j < 2 * G; ++j) Ft[j] *= ginv;
+    if (og_real_output)
+      for (int64_t j = 0; j < G; ++j) Ft[2 * j + 1] = 0.0;
     og_full_means(G, Ft, means);
     double dlogt = og_dlogt(N, t0, tk, k == 0 ? NULL : times + (int64_t)(k - 1) * N);
     og_identify_convergence(G, density, means, vmr, fp1, fp2, dlogt, eps, steps_low, steps_over,

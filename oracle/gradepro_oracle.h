@@ -102,11 +102,22 @@ int og_fft_nd(int ndims, const int64_t *n, double *data, int sign);
  * rounding envelope; mode 2 also perturbs lengths whose factors are all 4).
  * Not thread-safe: set it between estimates. */
 void og_set_fft_order(int mode);
-/* 1: the Dirac-sequence deposits of each resample are summed from the last
- * draw to the first (0: in draw order, as the reference does).  The same sums
- * in exact arithmetic; a second rounding-envelope probe, for comparisons with
- * binning that sums in another order (FP64 atomics).  Not thread-safe. */
-void og_set_deposit_order(int reversed);
+/* 1: the ladder keeps only the real part of each normalised inverse
+ * transform (a complex-to-real inverse: the exact result is real); 0 (the
+ * reference): |.| of the complex result, whose imaginary part is rounding
+ * noise.  A rounding-envelope probe.  Not thread-safe. */
+void og_set_real_output(int on);
+/* Deposit mode of the Dirac sequences.  0: FP64 sums in draw order, as the
+ * reference does.  1: the same sums from the last draw to the first (equal in
+ * exact arithmetic; a rounding-envelope probe for binning that sums in
+ * another order, FP64 atomics).  2: the fixed-point deterministic mode: each
+ * deposit rounded to llrint(term * scale), scale = 2^fb * n * prod h (squares:
+ * 2^fb * (n * prod h)^2), fb = og_fixed_frac_bits(n), summed exactly in 64-bit
+ * integers and scaled back by (double)sum * (1 / scale).  Not thread-safe. */
+void og_set_deposit_order(int mode);
+/* Fraction bits of the fixed-point deposits for n samples: 61 - (bits of n),
+ * so a cell (at most n whole draws) stays below 2^61. */
+int og_fixed_frac_bits(int64_t n);
 /* propagate_time_cpu! (FourierSpace.jl:215-262) on one complex field. vars may
  * be NULL (means-only form). */
 void og_propagate(const og_grid *g, const double *kgrid_axes, const double *t,

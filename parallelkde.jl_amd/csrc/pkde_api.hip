@@ -40,7 +40,14 @@ static int dalloc(void** p, size_t bytes, const char* what) {
   if (*p) return PKDE_OK;
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes == 0) bytes = 256;
-  return check_hip(hipMalloc(p, bytes), what);
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    // a failed allocation is reported here; it must not also surface as the
+    // launch error of a later, unrelated estimate
+    (void)hipGetLastError();
+  }
+  return check_hip(e, what);
 }
 
 static void dfree(void* p) {
@@ -134,12 +141,18 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
   const uint64_t units = (uint64_t)y_units(pl);
   if (per_rung == 0) return (uint64_t)fsize(pl) * 2 * rs * 2;  // :rot: the means pass only
-  uint64_t budget = pl->ctx->chunk_bytes;
+  uint64_t budget = pl->chunk_budget ? pl->chunk_budget : pl->ctx->chunk_bytes;
   if (!pl->ctx->chunk_fixed && pl->ctx->mem_total)
     budget = std::min<uint64_t>(budget, pl->ctx->mem_total / 32);
   uint64_t rungs = std::max<uint64_t>(1, budget / units / per_rung);
-  if (!pl->ctx->chunk_fixed)
-    rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
+  // at least 4 rungs when they fit in 1/8 of the device memory (2 rungs in
+  // 1/16 for a lane or slot plan, several of which share the device)
+  if (!pl->ctx->chunk_fixed) {
+    const bool lane = pl->chunk_budget != 0;
+    const uint64_t floor_r = std::min<uint64_t>(lane ? 2 : 4, pl->ctx->mem_total / (lane ? 16 : 8) /
+                                                                   units / per_rung);
+    rungs = std::max<uint64_t>(rungs, floor_r);
+  }
   const uint64_t S = (uint64_t)pl->S;
   rungs = std::min<uint64_t>(rungs, S);
   const uint64_t chunks = (S + rungs - 1) / rungs;
@@ -147,7 +160,18 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   return std::max<uint64_t>(rungs * per_rung, (uint64_t)fsize(pl) * 2 * rs * 2);
 }
 
-uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
+uint64_t plan_chunk_bytes(const pkde_plan* pl) {
+  return pl->g.N >= 2 ? (uint64_t)y_units(pl) * y_part_bytes(pl) : 0;
+}
+
+void plan_borrow_chunk(pkde_plan* pl, void* Y, uint64_t bytes) {
+  if (pl->d.Y && !pl->d.Y_borrowed) dfree(pl->d.Y);
+  pl->d.Y = Y;
+  pl->d.Y_elems = (int64_t)(bytes / (2 * real_size(pl->prec)));
+  pl->d.Y_borrowed = true;
+}
+
+uint64_t plan_bytes(const pkde_plan* pl, int64_t n, bool with_chunk) {
   const int64_t G = pl->g.G, F = 2 * (int64_t)pl->B + 1, fs = fsize(pl);
   const size_t rs = real_size(pl->prec);
   uint64_t b = 0;
@@ -159,8 +183,9 @@ uint64_t plan_bytes(const pkde_plan* pl, int64_t n) {
   if (G <= kConvTermsMaxG) b += (uint64_t)pl->S * G * 16;  // convergence terms
   b += (uint64_t)G * (2 + 2 + 8);                 // chosen, sel, density
   b += (uint64_t)pl->S * pl->g.ksum * 2 * rs;     // Gaussian tables
-  if (pl->g.N >= 2) b += (uint64_t)y_units(pl) * y_part_bytes(pl);
+  if (with_chunk) b += plan_chunk_bytes(pl);
   if (pl->fixed && pl->g.N <= 3) b += binsort_bytes(pl->g, n, pl->B);  // sorted binning
+  b += (uint64_t)pl->g.ksum * 2 * 12 + 4096;     // twiddles, Fourier coordinates, scalars
   return b;
 }
 
@@ -186,7 +211,7 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   if ((rc = dalloc(&d.acc, (size_t)F * G * 8, "binning accumulators"))) return rc;
   if ((rc = dalloc((void**)&d.specD, (size_t)F * fs * sizeof(cplx<double>), "spectra"))) return rc;
   if (!pl->prec && (rc = dalloc(&d.spec, (size_t)F * fs * sizeof(cplx<float>), "spectra fp32"))) return rc;
-  if (pl->g.N >= 2 && !d.Y) {
+  if (pl->g.N >= 2 && !d.Y && !d.Y_borrowed) {
     const uint64_t Y = (uint64_t)y_units(pl) * y_part_bytes(pl);
     if ((rc = dalloc(&d.Y, Y, "rung chunk buffer"))) return rc;
     d.Y_elems = (int64_t)(Y / (2 * rs));
@@ -244,9 +269,9 @@ static void graphs_clear(PlanGraphs& G);
 static void plan_free(pkde_plan* pl) {
   graphs_clear(pl->graphs);
   PlanBuffers& d = pl->d;
-  void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y, d.vmr, d.conv, d.chosen, d.sel, d.density, d.err,
-                d.bw, d.mean, d.partial, d.kfreq, d.times, d.dlogt, d.vscale, d.cV, d.params,
-                d.gM, d.gV, d.invk, d.binwork, d.twg64, d.twg32};
+  void* ps[] = {d.x, d.idx, d.acc, d.specD, d.spec, d.Y_borrowed ? nullptr : d.Y, d.vmr, d.conv,
+                d.chosen, d.sel, d.density, d.err, d.bw, d.mean, d.partial, d.kfreq, d.times,
+                d.dlogt, d.vscale, d.cV, d.params, d.gM, d.gV, d.invk, d.binwork, d.twg64, d.twg32};
   for (void* p : ps) dfree(p);
   d = PlanBuffers();
   for (auto& e : pl->ev)
@@ -618,6 +643,9 @@ int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_i
                  double* d_density, int16_t* d_chosen, hipStream_t s, bool timed) {
   if (n < 2) return fail(PKDE_INVALID_ARGUMENT, "need at least two samples");
   if (n >= (1ll << 31)) return fail(PKDE_INVALID_ARGUMENT, "more than 2^31-1 samples");
+  // launch errors are read back after this estimate's launches: drop any
+  // error an earlier call left behind
+  (void)hipGetLastError();
   int rc = plan_reserve(pl, n, false);
   if (rc) return rc;
   if (pl->S_from_data && pl->kind == 0 && (rc = ladder_from_data(pl, d_x, n, s))) return rc;
@@ -762,6 +790,134 @@ int autotune_plan(pkde_plan* pl, int reps, double* best_ms, double* default_ms) 
 
 }  // namespace pkde
 
+namespace pkde {
+
+// Host-side part of plan creation (initialize_estimator,
+// GradeProEstimator.jl:700-735, and the grid, Grids.jl:67-87): validates the
+// grid and keywords and fills every host field of the plan.  No device call.
+int plan_setup_host(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
+                    const double* hi, const double* h, const pkde_gradepro_params* params,
+                    pkde_plan* pl) {
+  if (!ctx || !pl || !n_per_axis || !lo || !hi || !h)
+    return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (ndims < 1 || ndims > PKDE_MAX_DIMS)
+    return fail(PKDE_UNSUPPORTED, "the :rocm estimator supports 1 to 6 dimensions");
+  pkde_gradepro_params p;
+  if (params)
+    p = *params;
+  else
+    pkde_gradepro_params_default(&p);
+  if (p.n_bootstraps < 2) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps must be at least 2");
+  if (p.n_bootstraps > 16383) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps too large");
+  pl->ctx = ctx;
+  pl->p = p;
+  GridDev& g = pl->g;
+  g.N = ndims;
+  g.G = 1;
+  g.ksum = 0;
+  for (int i = 0; i < PKDE_MAX_DIMS; ++i) {
+    g.n[i] = 1;
+    g.log2n[i] = 0;
+    g.lo[i] = g.hi[i] = g.h[i] = 0.0;
+    g.koff[i] = 0;
+  }
+  g.pow2 = 1;
+  for (int i = 0; i < ndims; ++i) {
+    if (n_per_axis[i] < 2 || n_per_axis[i] > 4096 || !factorize((int)n_per_axis[i], g.fac[i])) {
+        return fail(PKDE_UNSUPPORTED,
+                  "grid axis sizes must be in [2, 4096]");
+    }
+    if (!is_pow2(n_per_axis[i])) g.pow2 = 0;
+    factorize_wave((int)n_per_axis[i], g.facw[i]);
+    if (!(h[i] > 0) || !(hi[i] > lo[i]) || !isfinite(lo[i]) || !isfinite(hi[i])) {
+        return fail(PKDE_INVALID_ARGUMENT, "invalid grid bounds or spacing");
+    }
+    g.n[i] = (int)n_per_axis[i];
+    g.log2n[i] = is_pow2(n_per_axis[i]) ? ilog2(n_per_axis[i]) : 0;
+    g.lo[i] = lo[i];
+    g.hi[i] = hi[i];
+    g.h[i] = h[i];
+    g.koff[i] = g.ksum;
+    g.ksum += g.n[i];
+    g.G *= g.n[i];
+  }
+  g.H1 = g.n[0] / 2 + 1;
+  g.Q = g.G / g.n[0];
+  pl->B = p.n_bootstraps;
+  pl->prec = p.precision ? 1 : 0;
+  {
+    // tiled row layout: tile width = the k1 columns one block of the axis pass
+    // writing it holds (grid axis 2 in 2-D, axis 3 in 3-D; 256 lanes x E values)
+    const int E = pl->prec ? 8 : 16;
+    const int L = ndims == 3 ? g.n[2] : g.n[1];
+    g.pack = rows_path(g, pl->prec) ? 1 : 0;
+    g.Hc = g.pack ? (int)(n_per_axis[0] / 2) : g.H1;
+    int hp = 1;
+    while (hp < g.Hc) hp <<= 1;
+    g.TKy = ndims >= 2 ? std::min(hp, std::max(1, 256 * E / L)) : 1;
+    // axes that are not powers of two: 8-column tiles (128-byte FP64 rows) of
+    // the mixed-radix 2-D ladder
+    if (!g.pow2 && ndims >= 2) g.TKy = 8;
+    g.log2TKy = ilog2(g.TKy);
+    g.ktY = (g.Hc + g.TKy - 1) / g.TKy;
+    g.fsY = (int64_t)g.ktY * g.Q * g.TKy;
+  }
+  pl->fixed = p.deterministic_binning ? 1 : 0;
+  // defaults (GradeProEstimator.jl:541-550, 754-766)
+  pl->alpha_s = isnan(p.alpha_s) ? (ndims == 1 ? 0.0 : 0.06) : p.alpha_s;
+  pl->alpha_os = isnan(p.alpha_os) ? 0.1 : p.alpha_os;
+  pl->eps = isnan(p.eps) ? (ndims == 1 ? 2.0 : -0.5) : p.eps;
+  if (pl->alpha_s < 0 || pl->alpha_s > 1 || pl->alpha_os < 0 || pl->alpha_os > 1) {
+    return fail(PKDE_INVALID_ARGUMENT, "Fraction must be in the range [0, 1]");
+  }
+  pl->bw_given = true;
+  for (int i = 0; i < ndims; ++i) {
+    pl->bw_host[i] = p.bw_final[i];
+    pl->bw_given = pl->bw_given && isfinite(p.bw_final[i]);
+  }
+  if (pl->bw_given)
+    for (int i = 0; i < ndims; ++i)
+      if (!(pl->bw_host[i] > 0)) {
+            return fail(PKDE_INVALID_ARGUMENT, "bw_final must be positive");
+      }
+  // ladder length (GradeProEstimator.jl:860-905)
+  if (p.n_steps > 0) {
+    pl->steps_mode = 1;
+    pl->steps_or_step = (double)p.n_steps;
+    pl->S = p.n_steps + 1;
+  } else if (isfinite(p.bw_step) && !pl->bw_given) {
+    // the ladder length floor(bw_final / bw_step) waits for the samples
+    if (!(p.bw_step > 0.0)) {
+        return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
+    }
+    pl->steps_mode = 0;
+    pl->steps_or_step = p.bw_step;
+    pl->S = 251;
+    pl->S_from_data = true;
+  } else if (isfinite(p.bw_step)) {
+    double m = INFINITY;
+    for (int i = 0; i < ndims; ++i) {
+      if (p.bw_step > pl->bw_host[i] || p.bw_step <= 0.0) {
+            return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
+      }
+      m = std::min(m, floor(pl->bw_host[i] / p.bw_step));
+    }
+    pl->steps_mode = 0;
+    pl->steps_or_step = p.bw_step;
+    pl->S = (int)m + 1;
+  } else {
+    pl->steps_mode = 1;
+    pl->steps_or_step = 250.0;
+    pl->S = 251;
+  }
+  if (pl->S > 32000) {
+    return fail(PKDE_INVALID_ARGUMENT, "too many bandwidth steps");
+  }
+  return PKDE_OK;
+}
+
+}  // namespace pkde
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -838,6 +994,7 @@ void pkde_context_destroy(pkde_ctx* ctx) {
   for (auto& ln : ctx->lanes) {
     if (ln.stream) (void)hipStreamSynchronize(ln.stream);
     for (auto& kv : ln.plans) pkde_gradepro_plan_destroy(kv.second);
+    if (ln.Y) (void)hipFree(ln.Y);
     if (ln.stream) (void)hipStreamDestroy(ln.stream);
   }
   ctx->lanes.clear();
@@ -852,6 +1009,21 @@ void pkde_context_destroy(pkde_ctx* ctx) {
   if (ctx->tw64) (void)hipFree(ctx->tw64);
   if (ctx->tw32) (void)hipFree(ctx->tw32);
   delete ctx;
+}
+
+int pkde_context_release_batch(pkde_ctx* ctx) {
+  if (!ctx) return fail(PKDE_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  PKDE_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  for (auto& ln : ctx->lanes) {
+    if (ln.stream) PKDE_TRY(hipStreamSynchronize(ln.stream), "batch lane");
+    for (auto& kv : ln.plans) pkde_gradepro_plan_destroy(kv.second);
+    ln.plans.clear();
+    if (ln.Y) (void)hipFree(ln.Y);
+    ln.Y = nullptr;
+    ln.Y_bytes = 0;
+  }
+  return PKDE_OK;
 }
 
 int pkde_device_free_memory(pkde_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
@@ -902,127 +1074,15 @@ int pkde_gradepro_plan_create(pkde_ctx* ctx, int ndims, const int64_t* n_per_axi
   if (!ctx || !out || !n_per_axis || !lo || !hi || !h)
     return fail(PKDE_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
-  if (ndims < 1 || ndims > PKDE_MAX_DIMS)
-    return fail(PKDE_UNSUPPORTED, "the :rocm estimator supports 1 to 6 dimensions");
-  pkde_gradepro_params p;
-  if (params)
-    p = *params;
-  else
-    pkde_gradepro_params_default(&p);
-  if (p.n_bootstraps < 2) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps must be at least 2");
-  if (p.n_bootstraps > 16383) return fail(PKDE_INVALID_ARGUMENT, "n_bootstraps too large");
   pkde_plan* pl = new pkde_plan();
-  pl->ctx = ctx;
-  pl->p = p;
-  GridDev& g = pl->g;
-  g.N = ndims;
-  g.G = 1;
-  g.ksum = 0;
-  for (int i = 0; i < PKDE_MAX_DIMS; ++i) {
-    g.n[i] = 1;
-    g.log2n[i] = 0;
-    g.lo[i] = g.hi[i] = g.h[i] = 0.0;
-    g.koff[i] = 0;
-  }
-  g.pow2 = 1;
-  for (int i = 0; i < ndims; ++i) {
-    if (n_per_axis[i] < 2 || n_per_axis[i] > 4096 || !factorize((int)n_per_axis[i], g.fac[i])) {
-      delete pl;
-      return fail(PKDE_UNSUPPORTED,
-                  "grid axis sizes must be in [2, 4096]");
-    }
-    if (!is_pow2(n_per_axis[i])) g.pow2 = 0;
-    factorize_wave((int)n_per_axis[i], g.facw[i]);
-    if (!(h[i] > 0) || !(hi[i] > lo[i]) || !isfinite(lo[i]) || !isfinite(hi[i])) {
-      delete pl;
-      return fail(PKDE_INVALID_ARGUMENT, "invalid grid bounds or spacing");
-    }
-    g.n[i] = (int)n_per_axis[i];
-    g.log2n[i] = is_pow2(n_per_axis[i]) ? ilog2(n_per_axis[i]) : 0;
-    g.lo[i] = lo[i];
-    g.hi[i] = hi[i];
-    g.h[i] = h[i];
-    g.koff[i] = g.ksum;
-    g.ksum += g.n[i];
-    g.G *= g.n[i];
-  }
-  g.H1 = g.n[0] / 2 + 1;
-  g.Q = g.G / g.n[0];
-  pl->B = p.n_bootstraps;
-  pl->prec = p.precision ? 1 : 0;
   {
-    // tiled row layout: tile width = the k1 columns one block of the axis pass
-    // writing it holds (grid axis 2 in 2-D, axis 3 in 3-D; 256 lanes x E values)
-    const int E = pl->prec ? 8 : 16;
-    const int L = ndims == 3 ? g.n[2] : g.n[1];
-    g.pack = rows_path(g, pl->prec) ? 1 : 0;
-    g.Hc = g.pack ? (int)(n_per_axis[0] / 2) : g.H1;
-    int hp = 1;
-    while (hp < g.Hc) hp <<= 1;
-    g.TKy = ndims >= 2 ? std::min(hp, std::max(1, 256 * E / L)) : 1;
-    // axes that are not powers of two: 8-column tiles (128-byte FP64 rows) of
-    // the mixed-radix 2-D ladder
-    if (!g.pow2 && ndims >= 2) g.TKy = 8;
-    g.log2TKy = ilog2(g.TKy);
-    g.ktY = (g.Hc + g.TKy - 1) / g.TKy;
-    g.fsY = (int64_t)g.ktY * g.Q * g.TKy;
-  }
-  pl->fixed = p.deterministic_binning ? 1 : 0;
-  // defaults (GradeProEstimator.jl:541-550, 754-766)
-  pl->alpha_s = isnan(p.alpha_s) ? (ndims == 1 ? 0.0 : 0.06) : p.alpha_s;
-  pl->alpha_os = isnan(p.alpha_os) ? 0.1 : p.alpha_os;
-  pl->eps = isnan(p.eps) ? (ndims == 1 ? 2.0 : -0.5) : p.eps;
-  if (pl->alpha_s < 0 || pl->alpha_s > 1 || pl->alpha_os < 0 || pl->alpha_os > 1) {
-    delete pl;
-    return fail(PKDE_INVALID_ARGUMENT, "Fraction must be in the range [0, 1]");
-  }
-  pl->bw_given = true;
-  for (int i = 0; i < ndims; ++i) {
-    pl->bw_host[i] = p.bw_final[i];
-    pl->bw_given = pl->bw_given && isfinite(p.bw_final[i]);
-  }
-  if (pl->bw_given)
-    for (int i = 0; i < ndims; ++i)
-      if (!(pl->bw_host[i] > 0)) {
-        delete pl;
-        return fail(PKDE_INVALID_ARGUMENT, "bw_final must be positive");
-      }
-  // ladder length (GradeProEstimator.jl:860-905)
-  if (p.n_steps > 0) {
-    pl->steps_mode = 1;
-    pl->steps_or_step = (double)p.n_steps;
-    pl->S = p.n_steps + 1;
-  } else if (isfinite(p.bw_step) && !pl->bw_given) {
-    // the ladder length floor(bw_final / bw_step) waits for the samples
-    if (!(p.bw_step > 0.0)) {
+    const int hrc = plan_setup_host(ctx, ndims, n_per_axis, lo, hi, h, params, pl);
+    if (hrc) {
       delete pl;
-      return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
+      return hrc;
     }
-    pl->steps_mode = 0;
-    pl->steps_or_step = p.bw_step;
-    pl->S = 251;
-    pl->S_from_data = true;
-  } else if (isfinite(p.bw_step)) {
-    double m = INFINITY;
-    for (int i = 0; i < ndims; ++i) {
-      if (p.bw_step > pl->bw_host[i] || p.bw_step <= 0.0) {
-        delete pl;
-        return fail(PKDE_INVALID_ARGUMENT, "Invalid time step");
-      }
-      m = std::min(m, floor(pl->bw_host[i] / p.bw_step));
-    }
-    pl->steps_mode = 0;
-    pl->steps_or_step = p.bw_step;
-    pl->S = (int)m + 1;
-  } else {
-    pl->steps_mode = 1;
-    pl->steps_or_step = 250.0;
-    pl->S = 251;
   }
-  if (pl->S > 32000) {
-    delete pl;
-    return fail(PKDE_INVALID_ARGUMENT, "too many bandwidth steps");
-  }
+  const GridDev& g = pl->g;
   int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
   // Fourier coordinates 2pi*fftfreq(n, 1/h) (Grids.jl:225-231)
   std::vector<double> kf(g.ksum);
@@ -1220,8 +1280,10 @@ int pkde_gradepro_debug_enable(pkde_plan* pl, int enable) {
 
 int pkde_gradepro_debug_vmr(pkde_plan* pl, double* vmr_out, int64_t capacity) {
   if (!pl || !vmr_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
-  const int64_t need = (int64_t)pl->S * pl->g.G;
-  if (capacity < need) return fail(PKDE_INVALID_ARGUMENT, "vmr buffer too small");
+  // whole rungs only: the first min(capacity / G, S) rungs of the table
+  const int64_t G = pl->g.G;
+  if (capacity < G) return fail(PKDE_INVALID_ARGUMENT, "vmr buffer holds less than one rung");
+  const int64_t need = std::min<int64_t>(capacity / G, (int64_t)pl->S) * G;
   if (!pl->d.vmr) return fail(PKDE_INVALID_ARGUMENT, "no estimate has run");
   if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
   PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
@@ -1314,11 +1376,13 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
         const double two_fb = ldexp(1.0, fb);
         const double scM = (double)n * hp * two_fb;
         const double scV = ((double)n * hp) * ((double)n * hp) * two_fb;
+        // the conversion the forward transform applies (k_forward_axis1)
+        const double iM = 1.0 / scM, iV = 1.0 / scV;
         const long long* q = (const long long*)host.data();
         for (int64_t i = 0; i < (int64_t)B * G; ++i) {
-          sums_out[i] = deterministic ? (double)q[i] / scM : host[i];
+          sums_out[i] = deterministic ? (double)q[i] * iM : host[i];
           if (squares_out)
-            squares_out[i] = deterministic ? (double)q[(int64_t)B * G + i] / scV : host[(int64_t)B * G + i];
+            squares_out[i] = deterministic ? (double)q[(int64_t)B * G + i] * iV : host[(int64_t)B * G + i];
         }
       }
     }

@@ -33,6 +33,8 @@ struct pkde_batch_lane {
   hipStream_t stream = nullptr;
   std::vector<std::pair<std::vector<unsigned char>, pkde_plan*>> plans;
   double work = 0.0;
+  void* Y = nullptr;  // the lane's rung-chunk buffer, lent to its plans
+  uint64_t Y_bytes = 0;
 };
 
 struct pkde_ctx {
@@ -43,6 +45,9 @@ struct pkde_ctx {
   std::mutex mu;
   uint64_t chunk_bytes = 8ull << 30;  // per-plan budget of the rung-chunk buffer
   bool chunk_fixed = false;           // budget set by PKDE_CHUNK_BYTES
+  // budget of one stream lane's (batched form) or server slot's shared
+  // chunk buffer
+  uint64_t lane_chunk_bytes = 4ull << 30;
   uint64_t mem_total = 0;             // device memory, bytes
   // batched form
   std::vector<pkde_batch_lane> lanes;
@@ -78,6 +83,10 @@ struct PlanBuffers {
   void* spec = nullptr;           // FP32 copy (precision 0)
   void* Y = nullptr;              // rung-chunk buffer
   int64_t Y_elems = 0;
+  // Y belongs to the stream lane or server slot the plan runs on (plans of
+  // one stream run one after another, so they share one chunk buffer); the
+  // plan neither allocates nor frees it
+  bool Y_borrowed = false;
   double* vmr = nullptr;  // S x G
   double* conv = nullptr;  // S x G x 2 convergence terms (small grids only)
   int16_t* chosen = nullptr;
@@ -163,6 +172,9 @@ struct pkde_plan {
   pkde::TuneShape tune;
   int tune_gen = -1;
   int tune_S = -1;
+  // chunk budget of this plan (0: the context's); lane and slot plans take
+  // the lane budget
+  uint64_t chunk_budget = 0;
 };
 
 namespace pkde {
@@ -170,5 +182,17 @@ namespace pkde {
 int plan_enqueue(pkde_plan* pl, const double* d_x, int64_t n, const int32_t* d_idx,
                  double* d_density, int16_t* d_chosen, hipStream_t s, bool timed);
 int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx);
-uint64_t plan_bytes(const pkde_plan* pl, int64_t n);
+// Device bytes of a plan for n samples; with_chunk = false leaves out the
+// rung-chunk buffer (a lane or slot buffer lent to the plan)
+uint64_t plan_bytes(const pkde_plan* pl, int64_t n, bool with_chunk = true);
+// Bytes of the rung-chunk buffer the plan uses (0 on 1-D grids)
+uint64_t plan_chunk_bytes(const pkde_plan* pl);
+// Host-side part of plan creation: validates the grid and keywords and fills
+// every host field; allocates nothing on the device.  The plan's device
+// tables are then made by plan_device_init.
+int plan_setup_host(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, const double* lo,
+                    const double* hi, const double* h, const pkde_gradepro_params* params,
+                    pkde_plan* pl);
+// Lends a stream's chunk buffer to the plan (bytes >= plan_chunk_bytes()).
+void plan_borrow_chunk(pkde_plan* pl, void* Y, uint64_t bytes);
 }  // namespace pkde

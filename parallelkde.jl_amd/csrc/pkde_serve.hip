@@ -2,17 +2,28 @@
 //
 // Batched: a list of independent requests is spread over stream lanes
 // (longest-processing-time first) and enqueued with no host synchronisation
-// until the end, so small estimates run concurrently.  Each lane keeps its
-// plans (device buffers) resident between calls; host data moves through a
-// pinned staging buffer so asynchronous copies never block the host.
+// until the end, so small estimates run concurrently.  Each lane keeps one
+// plan per grid shape and keywords (the bootstrap seed is not part of the
+// key: it is a launch argument) resident between calls, and one rung-chunk
+// buffer that its plans borrow in turn (estimates on one stream run one after
+// another).  Host data moves through a pinned staging buffer so asynchronous
+// copies never block the host.
 //
 // Serving: a worker thread runs continuous batching over `max_inflight`
-// device streams.  A request is admitted as soon as a stream slot is free and
-// its device footprint fits the memory budget; admitted requests overlap on
-// the GPU.  Each slot keeps its plan (grid tables, spectra, rung buffers)
-// resident and reuses it for the next request of the same shape.  Results
-// do not depend on co-scheduled work: binning is fixed-point (order
-// independent) and every other reduction has a fixed order.
+// device streams (slots).  A request's device footprint is computed on the
+// host when it is submitted; a request whose footprint alone exceeds the
+// memory budget is rejected with PKDE_OUT_OF_MEMORY and the others go on.  A
+// request is admitted as soon as a slot is free and the memory it still
+// needs (a new plan for its shape, a larger chunk buffer for the slot) fits
+// the budget next to everything resident; idle plans are evicted, least
+// recently used first, to make room.  Plans live in a pool keyed by shape:
+// a request takes an idle plan of its shape, returns it when done, and a
+// plan is never freed while a stream uses it.  Admitted requests overlap on
+// the GPU.  Results do not depend on what else is in flight: binning is
+// fixed-point (order independent) and every other reduction has a fixed
+// order.  The memory model extends get_necessary_memory / check_memory
+// (GradeProEstimator.jl:1091-1112) from one estimate to the resident set of
+// a serving loop.
 #include <string.h>
 
 #include <algorithm>
@@ -38,6 +49,9 @@ struct PlanKey {
   bool operator==(const PlanKey& o) const { return memcmp(this, &o, sizeof(PlanKey)) == 0; }
 };
 
+// Grid and keywords; the Philox seed of the bootstrap draws is left out (it
+// is passed to each estimate), so requests that differ only in their seed
+// share plans.
 PlanKey key_of(const pkde_request& r) {
   PlanKey k;
   memset(&k, 0, sizeof(k));
@@ -49,11 +63,14 @@ PlanKey key_of(const pkde_request& r) {
     k.h[i] = r.h[i];
   }
   k.p = r.params;
+  k.p.rng_seed = 0;
   return k;
 }
 
-int make_plan(pkde_ctx* ctx, const pkde_request& r, pkde_plan** out) {
-  return pkde_gradepro_plan_create(ctx, r.ndims, r.n_per_axis, r.lo, r.hi, r.h, &r.params, out);
+std::vector<unsigned char> key_bytes(const pkde_request& r) {
+  PlanKey k = key_of(r);
+  const unsigned char* p = (const unsigned char*)&k;
+  return std::vector<unsigned char>(p, p + sizeof(k));
 }
 
 double now_ms() {
@@ -61,32 +78,77 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+int64_t pow2_ceil(int64_t v) {
+  int64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Host-side footprint of a request (no device call): the plan's buffers for
+// n samples and the chunk buffer its stream must lend it under the lane
+// budget.
+struct Footprint {
+  int status = PKDE_OK;
+  uint64_t plan = 0;   // plan buffers, chunk buffer excluded
+  uint64_t chunk = 0;  // the lent chunk buffer
+  int64_t G = 0;
+  double work = 0.0;   // estimated device work (ladder + binning)
+};
+
+Footprint footprint_of(pkde_ctx* ctx, const pkde_request& r, int64_t n_cap) {
+  Footprint f;
+  pkde_plan hp;
+  f.status = plan_setup_host(ctx, r.ndims, r.n_per_axis, r.lo, r.hi, r.h, &r.params, &hp);
+  if (f.status) return f;
+  hp.chunk_budget = ctx->lane_chunk_bytes;
+  f.plan = plan_bytes(&hp, n_cap, false);
+  f.chunk = plan_chunk_bytes(&hp);
+  f.G = hp.g.G;
+  f.work = (double)hp.g.G * hp.S * hp.B + (double)r.n * hp.B * (double)(1 << r.ndims);
+  return f;
+}
+
+int check_request(const pkde_request& r) {
+  if (!r.samples || !r.density_out || r.n < 2 || r.ndims < 1 || r.ndims > PKDE_MAX_DIMS)
+    return fail(PKDE_INVALID_ARGUMENT, "invalid request");
+  return PKDE_OK;
+}
+
+// A plan for a stream lane or slot: the lane chunk budget, the lane's chunk
+// buffer lent to it.
+int make_lane_plan(pkde_ctx* ctx, const pkde_request& r, pkde_plan** out) {
+  const int rc =
+      pkde_gradepro_plan_create(ctx, r.ndims, r.n_per_axis, r.lo, r.hi, r.h, &r.params, out);
+  if (rc == PKDE_OK) (*out)->chunk_budget = ctx->lane_chunk_bytes;
+  return rc;
+}
+
+// Grows a stream's chunk buffer to `need` bytes (the stream is idle).
+int grow_chunk(void*& Y, uint64_t& Y_bytes, uint64_t need, const char* what) {
+  if (need <= Y_bytes) return PKDE_OK;
+  if (Y) (void)hipFree(Y);
+  Y = nullptr;
+  Y_bytes = 0;
+  const hipError_t e = hipMalloc(&Y, (size_t)need);
+  if (e != hipSuccess) {
+    Y = nullptr;
+    (void)hipGetLastError();
+    return check_hip(e, what);
+  }
+  Y_bytes = need;
+  return PKDE_OK;
+}
+
 }  // namespace
 
-// Estimated device work of a request: the ladder (grid points x rungs x
-// bootstraps) plus binning (draws x corners).
-static double request_work(const pkde_request& r) {
-  double G = 1.0;
-  for (int i = 0; i < r.ndims; ++i) G *= (double)r.n_per_axis[i];
-  const double B = r.params.n_bootstraps > 0 ? r.params.n_bootstraps : 100;
-  const double S = r.params.n_steps > 0 ? r.params.n_steps + 1 : 251;
-  return G * S * B + (double)r.n * B * (double)(1 << r.ndims);
-}
-
-static std::vector<unsigned char> key_bytes(const pkde_request& r) {
-  PlanKey k = key_of(r);
-  const unsigned char* p = (const unsigned char*)&k;
-  return std::vector<unsigned char>(p, p + sizeof(k));
-}
-
+// ------------------------------------------------------------------- batch
 extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const pkde_request* reqs,
                                             int32_t* status_out) {
   if (!ctx || count < 0 || (count > 0 && !reqs)) return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
-  // stream lanes: requests are spread over independent streams so that small
-  // estimates run concurrently; each lane keeps its plans resident
+  (void)hipGetLastError();
   int nl = 8;
   if (const char* e = getenv("PKDE_BATCH_STREAMS")) nl = std::max(1, std::min(16, atoi(e)));
   while ((int)ctx->lanes.size() < nl) {
@@ -94,6 +156,15 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
     rc = check_hip(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking), "stream");
     if (rc) return rc;
     ctx->lanes.push_back(ln);
+  }
+  std::vector<int> st(count, PKDE_OK);
+  std::vector<Footprint> fp(count);
+  for (int i = 0; i < count; ++i) {
+    st[i] = check_request(reqs[i]);
+    if (st[i] == PKDE_OK) {
+      fp[i] = footprint_of(ctx, reqs[i], reqs[i].n);
+      st[i] = fp[i].status;
+    }
   }
   // pinned staging: samples in, density / chosen / flag out
   std::vector<size_t> off_x(count), off_d(count), off_c(count), off_e(count);
@@ -105,11 +176,11 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
   };
   for (int i = 0; i < count; ++i) {
     const pkde_request& r = reqs[i];
-    int64_t G = 1;
-    for (int a = 0; a < r.ndims && a < PKDE_MAX_DIMS; ++a) G *= r.n_per_axis[a];
-    off_x[i] = take((size_t)std::max<int64_t>(r.n, 0) * std::max(r.ndims, 0) * 8);
-    off_d[i] = take((size_t)std::max<int64_t>(G, 0) * 8);
-    off_c[i] = take((size_t)std::max<int64_t>(G, 0) * 2);
+    const int64_t G = st[i] == PKDE_OK ? fp[i].G : 0;
+    const int64_t n = st[i] == PKDE_OK ? r.n : 0;
+    off_x[i] = take((size_t)n * r.ndims * 8);
+    off_d[i] = take((size_t)G * 8);
+    off_c[i] = take((size_t)G * 2);
     off_e[i] = take(4);
   }
   if (total > ctx->stage_bytes) {
@@ -121,44 +192,92 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
     ctx->stage_bytes = total;
   }
   // longest-processing-time assignment of requests to lanes
-  std::vector<int> order(count), lane_of(count, 0);
-  for (int i = 0; i < count; ++i) order[i] = i;
-  std::sort(order.begin(), order.end(),
-            [&](int x, int y) { return request_work(reqs[x]) > request_work(reqs[y]); });
+  std::vector<int> order;
+  for (int i = 0; i < count; ++i)
+    if (st[i] == PKDE_OK) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return fp[x].work > fp[y].work; });
+  std::vector<int> lane_of(count, 0);
   for (auto& ln : ctx->lanes) ln.work = 0.0;
   for (int i : order) {
     int best = 0;
     for (int l = 1; l < nl; ++l)
       if (ctx->lanes[l].work < ctx->lanes[best].work) best = l;
     lane_of[i] = best;
-    ctx->lanes[best].work += request_work(reqs[i]);
+    ctx->lanes[best].work += fp[i].work;
   }
-  std::vector<int> st(count, PKDE_OK);
+  // Every lane is idle here (the previous call synchronised them).  When
+  // the plans this batch still has to create do not fit the free memory,
+  // the lanes drop the plans this batch does not use.
+  std::vector<std::vector<unsigned char>> keys(count);
   std::vector<pkde_plan*> used(count, nullptr);
-  int first_rc = PKDE_OK;
+  {
+    uint64_t need = 0;
+    std::vector<uint64_t> lane_chunk(nl, 0);
+    for (int i : order) {
+      keys[i] = key_bytes(reqs[i]);
+      bool have = false;
+      for (auto& kv : ctx->lanes[lane_of[i]].plans) have = have || kv.first == keys[i];
+      if (!have) need += fp[i].plan;
+      lane_chunk[lane_of[i]] = std::max(lane_chunk[lane_of[i]], fp[i].chunk);
+    }
+    for (int l = 0; l < nl; ++l)
+      if (lane_chunk[l] > ctx->lanes[l].Y_bytes) need += lane_chunk[l] - ctx->lanes[l].Y_bytes;
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    if (need + (1ull << 30) > fr) {
+      for (int l = 0; l < nl; ++l) {
+        auto& pls = ctx->lanes[l].plans;
+        for (size_t j = 0; j < pls.size();) {
+          bool in_batch = false;
+          for (int i : order) in_batch = in_batch || (lane_of[i] == l && keys[i] == pls[j].first);
+          if (in_batch) {
+            ++j;
+            continue;
+          }
+          pkde_gradepro_plan_destroy(pls[j].second);
+          pls.erase(pls.begin() + j);
+        }
+      }
+    }
+    for (int l = 0; l < nl; ++l) {
+      pkde_batch_lane& ln = ctx->lanes[l];
+      const int e = grow_chunk(ln.Y, ln.Y_bytes, lane_chunk[l], "lane chunk buffer");
+      if (e)
+        for (int i : order)
+          if (lane_of[i] == l) st[i] = e;
+    }
+  }
+  // plans, reserved for the largest request of their shape in this batch
   for (int i : order) {
-    const pkde_request& r = reqs[i];
+    if (st[i]) continue;
     pkde_batch_lane& ln = ctx->lanes[lane_of[i]];
-    hipStream_t s = ln.stream;
-    if (!r.samples || !r.density_out || r.n < 2 || r.ndims < 1 || r.ndims > PKDE_MAX_DIMS) {
-      st[i] = fail(PKDE_INVALID_ARGUMENT, "invalid request");
-      continue;
-    }
     pkde_plan* pl = nullptr;
-    std::vector<unsigned char> k = key_bytes(r);
     for (auto& kv : ln.plans)
-      if (kv.first == k) pl = kv.second;
+      if (kv.first == keys[i]) pl = kv.second;
     if (!pl) {
-      st[i] = make_plan(ctx, r, &pl);
+      st[i] = make_lane_plan(ctx, reqs[i], &pl);
       if (st[i]) continue;
-      ln.plans.push_back({k, pl});
+      ln.plans.push_back({keys[i], pl});
     }
+    if (pl->g.N >= 2) plan_borrow_chunk(pl, ln.Y, ln.Y_bytes);
     used[i] = pl;
+  }
+  for (int i : order) {
+    if (st[i]) continue;
+    int64_t nmax = reqs[i].n;
+    for (int j : order)
+      if (used[j] == used[i]) nmax = std::max<int64_t>(nmax, reqs[j].n);
+    st[i] = plan_reserve(used[i], nmax, false);
+  }
+  for (int i : order) {
+    if (st[i]) continue;
+    const pkde_request& r = reqs[i];
+    pkde_plan* pl = used[i];
+    hipStream_t s = ctx->lanes[lane_of[i]].stream;
+    pl->p.rng_seed = r.params.rng_seed;
     memcpy(ctx->stage + off_x[i], r.samples, (size_t)r.n * r.ndims * 8);
-    st[i] = plan_reserve(pl, r.n, false);
-    if (!st[i])
-      st[i] = check_hip(hipMemcpyAsync(pl->d.x, ctx->stage + off_x[i], (size_t)r.n * r.ndims * 8,
-                                       hipMemcpyHostToDevice, s), "upload");
+    st[i] = check_hip(hipMemcpyAsync(pl->d.x, ctx->stage + off_x[i], (size_t)r.n * r.ndims * 8,
+                                     hipMemcpyHostToDevice, s), "upload");
     if (!st[i]) st[i] = plan_enqueue(pl, pl->d.x, r.n, nullptr, pl->d.density, pl->d.chosen, s, false);
     if (!st[i])
       st[i] = check_hip(hipMemcpyAsync(ctx->stage + off_d[i], pl->d.density, (size_t)pl->g.G * 8,
@@ -174,6 +293,7 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
     int e = check_hip(hipStreamSynchronize(ctx->lanes[l].stream), "batch");
     if (e && !rc) rc = e;
   }
+  int first_rc = PKDE_OK;
   for (int i = 0; i < count; ++i) {
     if (st[i] == PKDE_OK && used[i]) {
       const int64_t G = used[i]->g.G;
@@ -192,27 +312,52 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
   return rc ? rc : first_rc;
 }
 
+extern "C" int pkde_gradepro_request_footprint(pkde_ctx* ctx, const pkde_request* req,
+                                               uint64_t* bytes_out) {
+  if (!ctx || !req || !bytes_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  const int rc = check_request(*req);
+  if (rc) return rc;
+  const Footprint f = footprint_of(ctx, *req, req->n);
+  if (f.status) return f.status;
+  *bytes_out = f.plan + f.chunk;
+  return PKDE_OK;
+}
+
 // ------------------------------------------------------------------ server
 struct Job {
   uint64_t ticket = 0;
   pkde_request req{};
+  PlanKey key{};
   std::vector<double> samples;
   int status = PKDE_PENDING;
   bool done = false;
   double t_submit = 0, t_done = 0;
-  uint64_t bytes = 0;
+  Footprint fp;
+  int64_t n_cap = 0;  // samples the plan is reserved for (a power of two)
+};
+
+struct PoolPlan {
+  pkde_plan* pl = nullptr;
+  PlanKey key{};
+  uint64_t bytes = 0;  // resident device bytes (chunk buffer excluded)
+  int64_t n_cap = 0;
+  bool busy = false;
+  double last_used = 0;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  pkde_plan* plan = nullptr;
-  PlanKey key{};
+  void* Y = nullptr;  // the slot's chunk buffer, lent to the plan it runs
+  uint64_t Y_bytes = 0;
+  PoolPlan* pp = nullptr;
   std::shared_ptr<Job> job;
   int* err_host = nullptr;  // pinned
   double* dens_host = nullptr;
   int16_t* chosen_host = nullptr;
   int64_t host_cap = 0;
+  double* x_host = nullptr;  // pinned samples staging
+  int64_t x_cap = 0;
 };
 
 struct pkde_server {
@@ -225,70 +370,157 @@ struct pkde_server {
   std::deque<std::shared_ptr<Job>> queue;
   std::map<uint64_t, std::shared_ptr<Job>> jobs;
   std::vector<Slot> slots;
+  std::vector<std::unique_ptr<PoolPlan>> pool;  // worker thread only
   uint64_t next_ticket = 1;
-  uint64_t completed = 0, rejected = 0, bytes_in_use = 0;
+  uint64_t completed = 0, rejected = 0;
+  uint64_t bytes_in_use = 0;  // pool plans + slot chunk buffers (read under mu)
   std::vector<double> lat;
   bool stop = false;
 };
 
+static uint64_t resident_bytes(const pkde_server* sv) {
+  uint64_t b = 0;
+  for (const auto& p : sv->pool) b += p->bytes;
+  for (const auto& sl : sv->slots) b += sl.Y_bytes;
+  return b;
+}
+
+static void publish_bytes(pkde_server* sv) {
+  const uint64_t b = resident_bytes(sv);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->bytes_in_use = b;
+}
+
+static void complete(pkde_server* sv, const std::shared_ptr<Job>& job, int status, bool rejected) {
+  std::lock_guard<std::mutex> lk(sv->mu);
+  job->status = status;
+  job->done = true;
+  job->t_done = now_ms();
+  if (status == PKDE_OK) {
+    sv->lat.push_back(job->t_done - job->t_submit);
+    sv->completed++;
+  }
+  if (rejected) sv->rejected++;
+  sv->done_cv.notify_all();
+}
+
 static void finish_job(pkde_server* sv, Slot& sl) {
   auto job = sl.job;
-  const int64_t G = sl.plan->g.G;
+  const int64_t G = sl.pp->pl->g.G;
   int st = PKDE_OK;
-  if (*sl.err_host & 1) st = PKDE_OUT_OF_GRID;
+  if (*sl.err_host & 1) st = fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid.");
   if (st == PKDE_OK) {
     memcpy(job->req.density_out, sl.dens_host, (size_t)G * 8);
     if (job->req.chosen_out) memcpy(job->req.chosen_out, sl.chosen_host, (size_t)G * 2);
   }
-  std::lock_guard<std::mutex> lk(sv->mu);
-  job->status = st;
-  job->done = true;
-  job->t_done = now_ms();
-  sv->lat.push_back(job->t_done - job->t_submit);
-  sv->completed++;
-  sv->bytes_in_use -= job->bytes;
+  sl.pp->busy = false;
+  sl.pp->last_used = now_ms();
+  sl.pp = nullptr;
   sl.job.reset();
-  sv->done_cv.notify_all();
+  complete(sv, job, st, false);
 }
 
-static int start_job(pkde_server* sv, Slot& sl, std::shared_ptr<Job> job) {
-  const pkde_request& r = job->req;
-  PlanKey k = key_of(r);
-  int st = PKDE_OK;
-  if (!sl.plan || !(sl.key == k)) {
-    if (sl.plan) pkde_gradepro_plan_destroy(sl.plan);
-    sl.plan = nullptr;
-    st = make_plan(sv->ctx, r, &sl.plan);
-    if (st) return st;
-    sl.key = k;
+// Frees idle pool plans, least recently used first (never `keep`), until
+// `need` more bytes fit the budget.  True when they fit.
+static bool make_room(pkde_server* sv, uint64_t need, const PoolPlan* keep) {
+  for (;;) {
+    if (resident_bytes(sv) + need <= sv->cfg.memory_budget) return true;
+    int victim = -1;
+    for (int i = 0; i < (int)sv->pool.size(); ++i) {
+      const PoolPlan* p = sv->pool[i].get();
+      if (p->busy || p == keep) continue;
+      if (victim < 0 || p->last_used < sv->pool[victim]->last_used) victim = i;
+    }
+    if (victim < 0) return false;
+    pkde_gradepro_plan_destroy(sv->pool[victim]->pl);
+    sv->pool.erase(sv->pool.begin() + victim);
   }
-  pkde_plan* pl = sl.plan;
-  const int64_t G = pl->g.G;
+}
+
+static int grow_host(Slot& sl, int64_t G, int64_t nx) {
+  int st = PKDE_OK;
   if (G > sl.host_cap) {
     if (sl.dens_host) (void)hipHostFree(sl.dens_host);
     if (sl.chosen_host) (void)hipHostFree(sl.chosen_host);
     sl.dens_host = nullptr;
     sl.chosen_host = nullptr;
+    sl.host_cap = 0;
     if ((st = check_hip(hipHostMalloc((void**)&sl.dens_host, (size_t)G * 8, hipHostMallocDefault), "pinned")))
       return st;
     if ((st = check_hip(hipHostMalloc((void**)&sl.chosen_host, (size_t)G * 2, hipHostMallocDefault), "pinned")))
       return st;
     sl.host_cap = G;
   }
-  if ((st = plan_reserve(pl, r.n, false))) return st;
-  hipStream_t s = sl.stream;
-  if ((st = check_hip(hipMemcpyAsync(pl->d.x, job->samples.data(), (size_t)r.n * r.ndims * 8,
-                                     hipMemcpyHostToDevice, s), "upload")))
-    return st;
-  if ((st = plan_enqueue(pl, pl->d.x, r.n, nullptr, pl->d.density, pl->d.chosen, s, false))) return st;
-  if ((st = check_hip(hipMemcpyAsync(sl.dens_host, pl->d.density, (size_t)G * 8, hipMemcpyDeviceToHost, s), "dl")))
-    return st;
-  if ((st = check_hip(hipMemcpyAsync(sl.chosen_host, pl->d.chosen, (size_t)G * 2, hipMemcpyDeviceToHost, s), "dl")))
-    return st;
-  if ((st = check_hip(hipMemcpyAsync(sl.err_host, pl->d.err, 4, hipMemcpyDeviceToHost, s), "dl"))) return st;
-  if ((st = check_hip(hipEventRecord(sl.done, s), "event"))) return st;
-  sl.job = job;
+  if (nx > sl.x_cap) {
+    if (sl.x_host) (void)hipHostFree(sl.x_host);
+    sl.x_host = nullptr;
+    sl.x_cap = 0;
+    if ((st = check_hip(hipHostMalloc((void**)&sl.x_host, (size_t)nx * 8, hipHostMallocDefault), "pinned")))
+      return st;
+    sl.x_cap = nx;
+  }
   return PKDE_OK;
+}
+
+// Tries to start the job on the free slot.  Returns 1 when started, 0 when it
+// must wait for memory, and -status when it failed.
+static int start_job(pkde_server* sv, Slot& sl, const std::shared_ptr<Job>& job) {
+  const pkde_request& r = job->req;
+  // an idle plan of the request's shape, the one reserved for the most
+  // samples first
+  PoolPlan* pp = nullptr;
+  for (auto& p : sv->pool)
+    if (!p->busy && p->key == job->key && (!pp || p->n_cap > pp->n_cap)) pp = p.get();
+  uint64_t need = sl.Y_bytes < job->fp.chunk ? job->fp.chunk - sl.Y_bytes : 0;
+  if (!pp)
+    need += job->fp.plan;
+  else if (pp->n_cap < job->n_cap)
+    need += job->fp.plan > pp->bytes ? job->fp.plan - pp->bytes : 0;
+  if (!make_room(sv, need, pp)) return 0;
+  int st = PKDE_OK;
+  if ((st = grow_chunk(sl.Y, sl.Y_bytes, job->fp.chunk, "slot chunk buffer"))) return -st;
+  if (!pp) {
+    auto np = std::make_unique<PoolPlan>();
+    pkde_plan* pl = nullptr;
+    if ((st = make_lane_plan(sv->ctx, r, &pl))) return -st;
+    np->pl = pl;
+    np->key = job->key;
+    pp = np.get();
+    sv->pool.push_back(std::move(np));
+  }
+  pkde_plan* pl = pp->pl;
+  // the slot's chunk buffer first, so that reserving never allocates one
+  if (pl->g.N >= 2) plan_borrow_chunk(pl, sl.Y, sl.Y_bytes);
+  if (pp->n_cap < job->n_cap) {
+    if ((st = plan_reserve(pl, job->n_cap, false))) return -st;
+    pp->n_cap = job->n_cap;
+    pp->bytes = std::max(pp->bytes, job->fp.plan);
+  }
+  const int64_t G = pl->g.G;
+  if ((st = grow_host(sl, G, r.n * r.ndims))) return -st;
+  pp->busy = true;
+  publish_bytes(sv);
+  hipStream_t s = sl.stream;
+  pl->p.rng_seed = r.params.rng_seed;
+  memcpy(sl.x_host, job->samples.data(), (size_t)r.n * r.ndims * 8);
+  st = check_hip(hipMemcpyAsync(pl->d.x, sl.x_host, (size_t)r.n * r.ndims * 8, hipMemcpyHostToDevice, s),
+                 "upload");
+  if (!st) st = plan_enqueue(pl, pl->d.x, r.n, nullptr, pl->d.density, pl->d.chosen, s, false);
+  if (!st)
+    st = check_hip(hipMemcpyAsync(sl.dens_host, pl->d.density, (size_t)G * 8, hipMemcpyDeviceToHost, s), "dl");
+  if (!st)
+    st = check_hip(hipMemcpyAsync(sl.chosen_host, pl->d.chosen, (size_t)G * 2, hipMemcpyDeviceToHost, s), "dl");
+  if (!st) st = check_hip(hipMemcpyAsync(sl.err_host, pl->d.err, 4, hipMemcpyDeviceToHost, s), "dl");
+  if (!st) st = check_hip(hipEventRecord(sl.done, s), "event");
+  if (st) {
+    // the stream may still read the plan's buffers: wait before reusing it
+    (void)hipStreamSynchronize(s);
+    pp->busy = false;
+    return -st;
+  }
+  sl.pp = pp;
+  sl.job = job;
+  return 1;
 }
 
 static void server_loop(pkde_server* sv) {
@@ -301,40 +533,38 @@ static void server_loop(pkde_server* sv) {
         finish_job(sv, sl);
         progressed = true;
       }
-    // admissions (continuous batching)
+    // admissions (continuous batching), first come first served
     for (auto& sl : sv->slots) {
       if (sl.job) continue;
       std::shared_ptr<Job> job;
       {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->queue.empty()) break;
-        auto& front = sv->queue.front();
-        if (front->bytes > sv->cfg.memory_budget) {
-          front->status = PKDE_OUT_OF_MEMORY;
-          front->done = true;
-          front->t_done = now_ms();
-          sv->rejected++;
+        job = sv->queue.front();
+      }
+      if (job->fp.plan + job->fp.chunk > sv->cfg.memory_budget) {
+        {
+          std::lock_guard<std::mutex> lk(sv->mu);
           sv->queue.pop_front();
-          sv->done_cv.notify_all();
-          progressed = true;
-          continue;
         }
-        if (sv->bytes_in_use + front->bytes > sv->cfg.memory_budget) break;
-        job = front;
-        sv->queue.pop_front();
-        sv->bytes_in_use += job->bytes;
+        complete(sv, job, PKDE_OUT_OF_MEMORY, true);
+        progressed = true;
+        continue;
       }
-      int st = start_job(sv, sl, job);
-      if (st != PKDE_OK) {
+      const int r = start_job(sv, sl, job);
+      if (r == 0) {
+        bool busy = false;
+        for (auto& o : sv->slots) busy = busy || (bool)o.job;
+        if (busy) break;  // wait until running requests release memory
+      }
+      {
         std::lock_guard<std::mutex> lk(sv->mu);
-        job->status = st;
-        job->done = true;
-        job->t_done = now_ms();
-        sv->bytes_in_use -= job->bytes;
-        sv->done_cv.notify_all();
+        sv->queue.pop_front();
       }
+      if (r <= 0) complete(sv, job, r < 0 ? -r : PKDE_OUT_OF_MEMORY, r == 0);
       progressed = true;
     }
+    publish_bytes(sv);
     std::unique_lock<std::mutex> lk(sv->mu);
     bool busy = false;
     for (auto& sl : sv->slots) busy = busy || (bool)sl.job;
@@ -378,18 +608,19 @@ int pkde_server_create(pkde_ctx* ctx, const pkde_server_cfg* cfg, pkde_server** 
 }
 
 int pkde_server_submit(pkde_server* sv, const pkde_request* req, uint64_t* ticket) {
-  if (!sv || !req || !ticket || !req->samples || !req->density_out || req->n < 2)
-    return fail(PKDE_INVALID_ARGUMENT, "invalid request");
+  if (!sv || !req || !ticket) return fail(PKDE_INVALID_ARGUMENT, "invalid request");
+  int st = check_request(*req);
+  if (st) return st;
   auto job = std::make_shared<Job>();
   job->req = *req;
+  job->key = key_of(*req);
+  // footprint on the host, for a plan reserved for the next power of two of
+  // samples (plans of a shape are reused by later requests)
+  job->n_cap = pow2_ceil(req->n);
+  job->fp = footprint_of(sv->ctx, *req, job->n_cap);
+  if (job->fp.status) return job->fp.status;
   job->samples.assign(req->samples, req->samples + (size_t)req->n * req->ndims);
   job->req.samples = job->samples.data();
-  // footprint from a host-only plan description
-  pkde_plan* probe = nullptr;
-  int st = make_plan(sv->ctx, *req, &probe);
-  if (st) return st;
-  job->bytes = plan_bytes(probe, req->n);
-  pkde_gradepro_plan_destroy(probe);
   std::lock_guard<std::mutex> lk(sv->mu);
   job->ticket = sv->next_ticket++;
   job->t_submit = now_ms();
@@ -409,6 +640,7 @@ int pkde_server_poll(pkde_server* sv, uint64_t ticket, int* done) {
   if (!it->second->done) return PKDE_PENDING;
   const int st = it->second->status;
   sv->jobs.erase(it);
+  if (st == PKDE_OUT_OF_MEMORY) set_error("Not enough memory for the request within the server's memory budget.");
   return st;
 }
 
@@ -420,6 +652,10 @@ int pkde_server_wait(pkde_server* sv, uint64_t ticket) {
   auto job = it->second;
   sv->done_cv.wait(lk, [&] { return job->done; });
   sv->jobs.erase(ticket);
+  if (job->status == PKDE_OUT_OF_MEMORY)
+    set_error("Not enough memory for the request within the server's memory budget.");
+  else if (job->status == PKDE_OUT_OF_GRID)
+    set_error("Data points lie beyond grid. Remove them or increase grid boundaries.");
   return job->status;
 }
 
@@ -456,14 +692,18 @@ void pkde_server_destroy(pkde_server* sv) {
   }
   if (sv->worker.joinable()) sv->worker.join();
   (void)hipSetDevice(sv->ctx->device);
-  for (auto& sl : sv->slots) {
+  for (auto& sl : sv->slots)
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.plan) pkde_gradepro_plan_destroy(sl.plan);
+  for (auto& p : sv->pool) pkde_gradepro_plan_destroy(p->pl);
+  sv->pool.clear();
+  for (auto& sl : sv->slots) {
+    if (sl.Y) (void)hipFree(sl.Y);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.err_host) (void)hipHostFree(sl.err_host);
     if (sl.dens_host) (void)hipHostFree(sl.dens_host);
     if (sl.chosen_host) (void)hipHostFree(sl.chosen_host);
+    if (sl.x_host) (void)hipHostFree(sl.x_host);
   }
   delete sv;
 }

@@ -61,6 +61,10 @@ function context()
     return _ctx[]
 end
 
+# true when the library loads and a device context can be created; the
+# device-typed constructors below fall back to :cpu otherwise, as the
+# reference does for :cuda (ParallelKDE.jl:120-128, Grids.jl:41-44,
+# KDEs.jl:68-71)
 function functional()
     try
         context()
@@ -68,6 +72,11 @@ function functional()
     catch
         return false
     end
+end
+
+function warn_fallback()
+    @warn "No functional ROCm device detected. Falling back to ':cpu'."
+    return nothing
 end
 
 # ------------------------------------------------------------------- grids
@@ -84,7 +93,13 @@ Grids.bounds(g::RocGrid) = Grids.bounds(g.grid)
 Grids.get_coordinates(g::RocGrid) = Grids.get_coordinates(g.grid)
 Grids.fftgrid(g::RocGrid) = Grids.fftgrid(g.grid)
 
-Grids.initialize_grid(::IsROCm, ranges; kwargs...) = RocGrid(Grids.Grid(ranges))
+function Grids.initialize_grid(::IsROCm, ranges; kwargs...)
+    if !functional()
+        warn_fallback()
+        return Grids.initialize_grid(Devices.IsCPU(), ranges)
+    end
+    return RocGrid(Grids.Grid(ranges))
+end
 
 # --------------------------------------------------------------------- KDEs
 # FP64 samples on the host (an FP32 copy could not reproduce the FP64 cell
@@ -100,6 +115,10 @@ KDEs.get_nsamples(k::RocKDE) = size(k.data, 2)
 KDEs.set_nan_density!(k::RocKDE) = (fill!(k.density, NaN); nothing)
 
 function KDEs.initialize_kde(::IsROCm, data, dims::NTuple{N}; T=Float64) where {N}
+    if !functional()
+        warn_fallback()
+        return KDEs.initialize_kde(Devices.IsCPU(), data, dims)
+    end
     m = data isa AbstractMatrix ? Matrix{Float64}(data) : reduce(hcat, Vector{Float64}.(data))
     size(m, 1) == N || throw(ArgumentError("Data must have $N dimensions"))
     return RocKDE{N,T,Float64}(m, fill(T(NaN), dims))
@@ -122,8 +141,11 @@ struct PkdeParams
 end
 
 # Mutable so that the finalizer is tied to the estimator object itself: the
-# plan is destroyed only once nothing references the estimator.
-mutable struct RocGradeProEstimator{N} <: DensityEstimators.AbstractGradeProEstimator{N,Float64,N + 1}
+# plan is destroyed only once nothing references the estimator.  M is the
+# rank of the bootstrapped statistics (the grid's N axes plus the bootstrap
+# axis), a type parameter of its own as in GradeProEstimator{N,T,M,P,S}
+# (GradeProEstimator.jl:673); constructors pass M = N + 1.
+mutable struct RocGradeProEstimator{N,M} <: DensityEstimators.AbstractGradeProEstimator{N,Float64,M}
     plan::Ptr{Cvoid}
 end
 
@@ -174,7 +196,7 @@ function DensityEstimators.initialize_estimator(
     check(ccall((:pkde_gradepro_plan_create, libpkde), Cint,
         (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{PkdeParams}, Ref{Ptr{Cvoid}}),
         context(), N, n, lo, hi, h, p, plan))
-    return finalizer(destroy!, RocGradeProEstimator{N}(plan[]))
+    return finalizer(destroy!, RocGradeProEstimator{N,N + 1}(plan[]))
 end
 
 # One ccall per estimate (GradeProEstimator.jl:968-1072 on the device).
@@ -182,8 +204,8 @@ end
 # ladder's launch shapes for this grid on the device (pkde_gradepro_autotune);
 # later estimates of the same grid, bootstraps, ladder and precision use the
 # fastest.  The density does not change.
-function DensityEstimators.estimate!(est::RocGradeProEstimator{N}, kde::RocKDE{N}; method::Symbol,
-                                     autotune::Bool=false, kwargs...) where {N}
+function DensityEstimators.estimate!(est::RocGradeProEstimator{N,M}, kde::RocKDE{N}; method::Symbol,
+                                     autotune::Bool=false, kwargs...) where {N,M}
     n = size(kde.data, 2)
     chosen = Array{Int16}(undef, size(kde.density))
     est.plan == C_NULL && throw(ArgumentError("estimator plan was destroyed"))

@@ -30,6 +30,7 @@ __all__ = [
     "spacings", "bounds", "low_bounds", "high_bounds", "initial_bandwidth", "fftgrid",
     "AVAILABLE_DEVICES", "DEVICE_IMPLEMENTATIONS", "DEFAULT_IMPLEMENTATIONS",
     "estimator_lookup", "GradeProResult", "RoTResult", "estimate_batch", "Server", "context",
+    "request_footprint", "release_batch_memory",
 ]
 
 
@@ -624,9 +625,41 @@ def autotune(est: DensityEstimation, reps=3):
 
 
 # ------------------------------------------------------------ batched form
-def estimate_batch(items, **common):
+def _request(x, grid, params):
+    r = _lib.Request()
+    r.ndims = grid.ndims
+    for a in range(grid.ndims):
+        r.n_per_axis[a], r.lo[a], r.hi[a], r.h[a] = grid.n[a], grid.lo[a], grid.hi[a], grid.h[a]
+    r.samples = x.ctypes.data
+    r.n = x.shape[0]
+    r.params = params
+    return r
+
+
+def request_footprint(data, grid: Grid, **kwargs):
+    """Device bytes one estimate of `data` on `grid` needs in the batched and
+    serving forms (its plan and the chunk buffer its stream lends it),
+    computed on the host (pkde_gradepro_request_footprint)."""
+    x = _as_samples(data)
+    r = _request(x, grid, _params(grid.ndims, **kwargs))
+    dummy = np.empty(1)
+    r.density_out = dummy.ctypes.data
+    out = _lib.C.c_uint64()
+    _raise(_lib.load().pkde_gradepro_request_footprint(context().handle, _lib.C.byref(r),
+                                                       _lib.C.byref(out)), "footprint")
+    return int(out.value)
+
+
+def release_batch_memory():
+    """Free the plans and chunk buffers the batched form keeps resident in the
+    device context between calls (pkde_context_release_batch)."""
+    _raise(_lib.load().pkde_context_release_batch(context().handle), "release")
+
+
+def estimate_batch(items, with_chosen=False, return_status=False, **common):
     """Many independent estimates in one submission.  items: iterable of
-    (data, grid) or (data, grid, kwargs).  Returns a list of densities."""
+    (data, grid) or (data, grid, kwargs).  Returns a list of densities, or of
+    (density, chosen rung) pairs with with_chosen=True."""
     L = _lib.load()
     ctx = context()
     reqs = (_lib.Request * len(items))()
@@ -648,15 +681,23 @@ def estimate_batch(items, **common):
         r.params = p
         dens = np.empty(int(np.prod(grid.shape)))
         r.density_out = dens.ctypes.data
-        r.chosen_out = None
+        chosen = np.empty(int(np.prod(grid.shape)), dtype=np.int16) if with_chosen else None
+        r.chosen_out = chosen.ctypes.data if with_chosen else None
         keep.append(x)
-        outs.append((dens, grid.shape))
+        outs.append((dens, chosen, grid.shape))
     status = np.zeros(len(items), dtype=np.int32)
     rc = L.pkde_gradepro_estimate_batch(ctx.handle, len(items), reqs, status.ctypes.data)
+    if return_status:
+        # per-request status codes; a failed request leaves its output unset
+        outs_r = [(d.reshape(shp, order="F"), None if c is None else c.reshape(shp, order="F"))
+                  for d, c, shp in outs]
+        return outs_r, [int(v) for v in status]
     for s in status:
         _raise(int(s), "batch request")
     _raise(rc, "batch")
-    return [d.reshape(shp, order="F") for d, shp in outs]
+    if with_chosen:
+        return [(d.reshape(shp, order="F"), c.reshape(shp, order="F")) for d, c, shp in outs]
+    return [d.reshape(shp, order="F") for d, _, shp in outs]
 
 
 class Server:

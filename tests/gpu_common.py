@@ -74,3 +74,79 @@ def compare(dens_gpu, chosen_gpu, dens_ref, chosen_ref, S, floor=1e-3):
     agree_sig = float(np.mean(cg[sig] == chosen_ref[sig]))
     return dict(rel_l1=rel_l1, rel_linf=rel_linf, agree_all=agree_all, agree_sig=agree_sig,
                 n_sig=int(sig.sum()))
+
+
+def box_draws(rng, n, d, lo, hi, kind, margin):
+    """N(0,I) or the mixture 1/2 N(-1.5*1, 0.5^2 I) + 1/2 N(+1.5*1, I) in d
+    dimensions; draws outside (lo, hi - margin) on any axis are redrawn."""
+    def draw(m):
+        if kind == "normal":
+            return rng.standard_normal((m, d))
+        comp = rng.integers(0, 2, m)[:, None]
+        return np.where(comp == 0, -1.5 + 0.5 * rng.standard_normal((m, d)),
+                        1.5 + rng.standard_normal((m, d)))
+    x = draw(n)
+    bad = np.any((x <= lo) | (x >= hi - margin), axis=1)
+    while bad.any():
+        x[bad] = draw(int(bad.sum()))
+        bad = np.any((x <= lo) | (x >= hi - margin), axis=1)
+    return x
+
+
+def batch_mix_items(count=64):
+    """The 64-dataset batched workload: dataset j (seed 1000 + j) has a
+    dimension uniform in {1, 2, 3}, a sample count log-uniform in [1e3, 2e5],
+    N(0,I) or 2-component mixture draws, and a grid of 1024 (1-D), 128^2
+    (2-D) or 64^3 (3-D) points over [-6, 6] per axis.  Returns
+    [(samples (n, d), [(lo, hi, points)] * d, seed)]."""
+    import math
+    items = []
+    for j in range(count):
+        rng = np.random.default_rng(1000 + j)
+        d = int(rng.integers(1, 4))
+        n = int(10 ** rng.uniform(3, math.log10(2e5)))
+        kind = "normal" if rng.random() < 0.5 else "mixture"
+        pts = {1: 1024, 2: 128, 3: 64}[d]
+        x = box_draws(rng, n, d, -6.0, 6.0, kind, 12.0 / (pts - 1) + 1e-9)
+        items.append((x, [(-6.0, 6.0, pts)] * d, 1000 + j))
+    return items
+
+
+def stream_requests(count, seed=7):
+    """Requests of the served Poisson stream: request r (seed
+    seed * 100000 + r) has a dimension uniform in {1, 2, 3}, a sample count
+    log-uniform in [1e3, 1e6] and per-axis points log-uniform in 256-1024
+    (1-D), 64-256 (2-D), 32-128 (3-D), rounded to the nearest power of two;
+    N(0,I) draws on [-6, 6]^d.  Returns [(samples, [(lo, hi, points)] * d,
+    seed)]."""
+    import math
+    reqs = []
+    for r in range(count):
+        s = seed * 100000 + r
+        rng = np.random.default_rng(s)
+        d = int(rng.integers(1, 4))
+        n = int(10 ** rng.uniform(3, 6))
+        lo_p, hi_p = {1: (256, 1024), 2: (64, 256), 3: (32, 128)}[d]
+        pts = int(2 ** round(rng.uniform(math.log2(lo_p), math.log2(hi_p))))
+        x = box_draws(rng, n, d, -6.0, 6.0, "normal", 12.0 / (pts - 1) + 1e-9)
+        reqs.append((x, [(-6.0, 6.0, pts)] * d, s))
+    return reqs
+
+
+def mass(density, spacing):
+    """Integral of a grid density: sum times the cell volume."""
+    return float(np.sum(density) * np.prod(spacing))
+
+
+def check_properties(density, chosen, S, spacing, mass_tol=1e-3):
+    """Size-independent properties of a :gradepro density: finite,
+    non-negative, unit mass (the binning puts mass 1 on the grid and every
+    Gaussian propagation keeps it), chosen rung in [0, S] (S = fallback)."""
+    d = np.asarray(density)
+    c = np.asarray(chosen)
+    assert np.all(np.isfinite(d)), "non-finite density"
+    assert np.all(d >= 0), "negative density"
+    m = mass(d, spacing)
+    assert abs(m - 1.0) <= mass_tol, m
+    assert c.min() >= 0 and c.max() <= S, (c.min(), c.max())
+    return m

@@ -84,6 +84,9 @@ def lib():
         L.og_fft_nd.argtypes = [C.c_int, C.POINTER(C.c_int64), _dp, C.c_int]
         L.og_set_fft_order.argtypes = [C.c_int]
         L.og_set_deposit_order.argtypes = [C.c_int]
+        L.og_set_real_output.argtypes = [C.c_int]
+        L.og_fixed_frac_bits.argtypes = [C.c_int64]
+        L.og_fixed_frac_bits.restype = C.c_int
         L.og_propagate.argtypes = [C.POINTER(OgGrid), _dp, _dp, _dp, _dp, C.c_void_p, _dp, C.c_void_p]
         L.og_scaled_vmr.argtypes = [C.c_int, C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, C.c_int64, _dp]
         L.og_full_means.argtypes = [C.c_int64, _dp, _dp]
@@ -175,12 +178,20 @@ def set_fft_order(mode: int):
     lib().og_set_fft_order(int(mode))
 
 
+def set_deposit_order(mode: int):
+    """Deposit mode of the oracle's binning (og_set_deposit_order): 0 FP64 in
+    draw order (the reference), 1 FP64 last draw first, 2 fixed point."""
+    lib().og_set_deposit_order(int(mode))
+
+
 def rounding_envelope(g: OgGrid, samples_col, idx_bs, deposits=False, **kw):
     """The oracle's estimate in its default summation orders and in others:
-    the FFT radices reversed, radix-2 passes, and (deposits=True) each
-    resample's deposits summed last draw first -- how far FP64 rounding alone
-    moves the reference's result on this input.  Returns (default result,
-    [results under the other orders])."""
+    the FFT radices reversed, radix-2 passes, the real part of each inverse
+    transform kept instead of |.| of the complex result (whose imaginary part
+    is rounding noise), and (deposits=True) each resample's deposits summed
+    last draw first and rounded to the fixed-point quantum -- how far rounding
+    alone moves the reference's result on this input.  Returns (default
+    result, [results under the other orders])."""
     r0 = gradepro(g, samples_col, idx_bs=idx_bs, **kw)
     others = []
     try:
@@ -188,12 +199,17 @@ def rounding_envelope(g: OgGrid, samples_col, idx_bs, deposits=False, **kw):
             set_fft_order(mode)
             others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
         set_fft_order(0)
+        lib().og_set_real_output(1)
+        others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
+        lib().og_set_real_output(0)
         if deposits:
-            lib().og_set_deposit_order(1)
-            others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
+            for mode in (1, 2):
+                lib().og_set_deposit_order(mode)
+                others.append(gradepro(g, samples_col, idx_bs=idx_bs, **kw))
     finally:
         set_fft_order(0)
         lib().og_set_deposit_order(0)
+        lib().og_set_real_output(0)
     return r0, others
 
 
@@ -256,8 +272,11 @@ def gradepro(g: OgGrid, samples_col, idx_bs=None, seed=0, n_bootstraps=100, n_st
     if idx_bs is not None:
         idx_arr = np.ascontiguousarray(idx_bs, dtype=np.int32)
         idx_ptr = idx_arr.ctypes.data
-    # number of rungs for table allocation
+    # number of rungs for table allocation (only the first max_rungs rungs
+    # are run and tabulated when max_rungs > 0)
     S_guess = (n_steps + 1) if n_steps > 0 else 4096
+    if max_rungs > 0:
+        S_guess = min(S_guess, max_rungs)
     vmr = np.zeros(S_guess * G) if tables else None
     means = np.zeros(S_guess * G) if tables else None
     rc = lib().og_gradepro_estimate(C.byref(g), s.ravel(), n, idx_ptr, seed, C.byref(p), density, chosen,
@@ -265,6 +284,7 @@ def gradepro(g: OgGrid, samples_col, idx_bs=None, seed=0, n_bootstraps=100, n_st
                                     means.ctypes.data if tables else None, max_rungs, C.byref(info))
     S = info.n_rungs
     if tables:
-        vmr = vmr[: S * G].reshape(S, G)
-        means = means[: S * G].reshape(S, G)
+        S_tab = min(S, S_guess)
+        vmr = vmr[: S_tab * G].reshape(S_tab, G)
+        means = means[: S_tab * G].reshape(S_tab, G)
     return OracleResult(density, chosen, vmr, means, info, rc)

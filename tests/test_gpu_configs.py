@@ -1,0 +1,359 @@
+"""GPU parity on the five benchmark workloads and on the timed branch.
+
+Every workload runs here at its full size through the C ABI:
+
+  * 1-D N(0,1), n = 1e4, 512 points over [-5, 5]          (whole estimate vs oracle)
+  * 2-D 3-component mixture, n = 1e5, 256^2 over [-6, 6]^2 (whole estimate vs oracle)
+  * 3-D correlated Gaussian + 10 % uniform, n = 1e6, 128^3 (properties on the
+    default ladder; whole estimate and log-VMR table vs oracle on a 5-rung
+    ladder at the same size)
+  * the 64-dataset batched mix (1-D/2-D items vs oracle; 64^3 items: properties)
+  * a served Poisson stream (alone == in the mix; properties; 1-D items vs oracle)
+
+The estimates use the in-kernel Philox bootstrap draws (bootstrap_idx =
+NULL), the branch bench.py times; the oracle draws the same indices with
+og_bootstrap_indices(seed) (Philox4x32-10 keyed by the seed, pinned by the
+Random123 known-answer vectors in test_oracle.py).
+
+Bars (FP64 mode):
+  * relative L1 density error <= 1e-5 against the FP64 oracle, or 4x the
+    oracle's own rounding envelope where that is larger (sparse cells whose
+    log-VMR is set by rounding noise decide their rung on that noise, in the
+    oracle as well; measured: 1-D 1024 points with n = 2.5e4, and 128^3 on a
+    5-rung ladder);
+  * chosen rung on points whose density is >= 1e-3 of the maximum: the same
+    rung as the oracle on >= 99.9 % of them, one point allowed where that is
+    less than one (or, where the oracle's own result moves further under
+    rounding alone, 3x its own disagreement);
+  * the rounding envelope: the oracle against itself with its FFT summed in
+    other radix orders and its deposits summed in another order or rounded
+    to the fixed-point quantum the GPU path's deterministic binning uses, and
+    with the real part of each inverse transform kept instead of |.| of the
+    complex result (the GPU path's inverse transforms are complex-to-real:
+    the exact result is real, and in cells where a resample is empty the
+    reference's |.| of rounding noise decides the log-VMR);
+  * chosen rung over all points (single-workload tests): the oracle itself,
+    with its FFT summed in other radix orders or its deposits in another
+    order, moves the rung of up to an eighth of the points (at the 256^2
+    workload it agrees with itself on 88 %: tail points whose log-VMR
+    differences sit at rounding level), so the GPU path may disagree on at
+    most 3x the oracle's own disagreement;
+  The measured figures are printed.
+  * mass: the density takes each point from its own rung, so its integral is
+    1 only up to how much the rungs differ (the means field of any one rung
+    integrates to 1 exactly): measured 0.98-1.03 on the full workloads and up
+    to 1.12 for 1e3 samples on 64^3; a sanity bound of 0.2 catches lost
+    normalisation factors (h^N, n, G), not rounding.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import gpu_common as W
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16  # the host cores a one-GPU box offers
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pk():
+    import parallelkde_rocm as pk
+    pk.context()
+    return pk
+
+
+def _spec(grid_ranges):
+    lo = [r[0] for r in grid_ranges]
+    hi = [r[1] for r in grid_ranges]
+    n = [r[2] for r in grid_ranges]
+    return lo, hi, n
+
+
+def _gpu_estimate(pk, x, grid_ranges, seed, B=100, steps=250, idx=None):
+    grid = pk.initialize_grid(grid_ranges)
+    plan = pk._Plan(grid, pk._params(grid.ndims, n_bootstraps=B, n_steps=steps, seed=seed))
+    try:
+        res = plan.estimate(np.ascontiguousarray(x), idx)
+    finally:
+        plan.close()
+    return res
+
+
+def _envelope_agreement(og, x, seed, S, B, steps):
+    """How far FP64 rounding alone moves the oracle on this input: the worst
+    relative L1 distance and the lowest chosen-rung agreement (all points,
+    significant points) between the default oracle and the oracle with its
+    FFT summed in other orders and its deposits summed in another order."""
+    idx = O.bootstrap_indices(seed, x.shape[0], B)
+    r0, others = O.rounding_envelope(og, x, idx, deposits=True, threads=THREADS, n_bootstraps=B,
+                                     n_steps=steps)
+    envs = [W.compare(r.density, r.chosen, r0.density, r0.chosen, S) for r in others]
+    return (max(e["rel_l1"] for e in envs), min(e["agree_all"] for e in envs),
+            min(e["agree_sig"] for e in envs))
+
+
+def _check_against_oracle(name, m, envelope, l1_bar=1e-5, sig_bar=0.999, check_all=True):
+    """relative L1 <= l1_bar and significant points on the oracle's rung >=
+    sig_bar; where the oracle's own result moves further under FP64 rounding
+    alone, relative L1 <= 4x and disagreement <= 3x the oracle's own (all
+    points and significant points)."""
+    say(f"  {name}: rel L1 {m['rel_l1']:.3e} rel Linf {m['rel_linf']:.3e} chosen agree "
+        f"{m['agree_all']:.5f} (sig {m['agree_sig']:.5f} of {m['n_sig']})")
+    # on small grids 0.1 % of the significant points is less than one point:
+    # one point may differ
+    sig_bar = min(sig_bar, 1.0 - 1.0 / max(m["n_sig"], 1))
+    if m["rel_l1"] <= l1_bar and m["agree_sig"] >= sig_bar and (
+            m["agree_all"] >= sig_bar or not check_all):
+        return
+    env_l1, env_all, env_sig = envelope()
+    say(f"  {name}: oracle rounding envelope rel L1 {env_l1:.3e} agreement {env_all:.5f} "
+        f"(sig {env_sig:.5f})")
+    assert m["rel_l1"] <= max(l1_bar, 4.0 * env_l1), (m, env_l1)
+    if check_all:
+        assert m["agree_all"] >= 1.0 - 3.0 * (1.0 - env_all), (m, env_all)
+    assert m["agree_sig"] >= min(sig_bar, 1.0 - 3.0 * (1.0 - env_sig)), (m, env_sig)
+
+
+# ------------------------------------------------- timed branch == explicit
+PHILOX_CASES = [
+    ("1d-512", [(-5.0, 5.0, 512)], lambda: W.gaussian_1d(10000, 1), 100, 250),
+    ("2d-128", [(-6.0, 6.0, 128)] * 2, lambda: W.mixture_2d(50000, 2), 100, 250),
+    ("3d-64", [(-6.0, 6.0, 64)] * 3, lambda: W.correlated_3d(50000, 3, lo=-6.0, hi=6.0), 50, 120),
+]
+
+
+@pytest.mark.parametrize("case", PHILOX_CASES, ids=[c[0] for c in PHILOX_CASES])
+def test_philox_branch_equals_explicit_indices(pk, case):
+    """bootstrap_idx = NULL draws the resamples in the binning kernels from
+    Philox4x32-10 keyed by rng_seed; the result is bit-identical to passing
+    the oracle's og_bootstrap_indices(seed) explicitly."""
+    name, ranges, gen, B, steps = case
+    x = gen()
+    seed = 0x1234_5678_9ABC
+    say(f"[philox] {name} B={B} S={steps + 1}")
+    a = _gpu_estimate(pk, x, ranges, seed, B, steps)
+    b = _gpu_estimate(pk, x, ranges, seed, B, steps, idx=O.bootstrap_indices(seed, x.shape[0], B))
+    assert np.array_equal(a.density, b.density)
+    assert np.array_equal(a.chosen_rung, b.chosen_rung)
+    # another key draws other resamples
+    c = _gpu_estimate(pk, x, ranges, seed + 1, B, steps)
+    assert not np.array_equal(a.density, c.density)
+
+
+# --------------------------------------------------------------- C1 and C2
+def test_c1_full_matches_oracle(pk):
+    """1-D N(0,1), n = 1e4, 512 points over [-5, 5], B = 100, S = 251,
+    in-kernel draws (seed 1)."""
+    ranges = [(-5.0, 5.0, 512)]
+    x = W.gaussian_1d(10000, 1)
+    seed = 1
+    res = _gpu_estimate(pk, x, ranges, seed)
+    og = O.make_grid(*_spec(ranges))
+    ref = O.gradepro(og, x, seed=seed, threads=THREADS)
+    assert ref.rc == 0
+    m = W.compare(res.density, res.chosen_rung, ref.density, ref.chosen, 251)
+    _check_against_oracle("C1 1-D 512", m, lambda: _envelope_agreement(og, x, seed, 251, 100, 250))
+    W.check_properties(res.density, res.chosen_rung, 251, pk.spacings(pk.initialize_grid(ranges)),
+                       mass_tol=0.2)
+
+
+@pytest.mark.timeout(600)
+def test_c2_full_matches_oracle(pk):
+    """2-D mixture, n = 1e5, 256^2 over [-6, 6]^2, B = 100, S = 251, in-kernel
+    draws (seed 2): the workload's whole estimate against the oracle."""
+    ranges = [(-6.0, 6.0, 256)] * 2
+    x = W.mixture_2d(100000, 2)
+    seed = 2
+    t = time.time()
+    res = _gpu_estimate(pk, x, ranges, seed)
+    say(f"[C2] gpu {res.ms_total:.1f} ms; oracle ({THREADS} threads) ...")
+    og = O.make_grid(*_spec(ranges))
+    ref = O.gradepro(og, x, seed=seed, threads=THREADS)
+    assert ref.rc == 0
+    say(f"[C2] oracle {time.time() - t:.1f} s")
+    m = W.compare(res.density, res.chosen_rung, ref.density, ref.chosen, 251)
+    _check_against_oracle("C2 2-D 256^2", m, lambda: _envelope_agreement(og, x, seed, 251, 100, 250))
+    mass = W.check_properties(res.density, res.chosen_rung, 251,
+                              pk.spacings(pk.initialize_grid(ranges)), mass_tol=0.2)
+    say(f"  C2 mass {mass:.6f}")
+
+
+# ---------------------------------------------------------------------- C3
+C3_RANGES = [(-4.0, 4.0, 128)] * 3
+
+
+@pytest.mark.timeout(900)
+def test_c3_full_properties(pk):
+    """3-D, n = 1e6 (90 % correlated Gaussian + 10 % uniform), 128^3 over
+    [-4, 4]^3, B = 100, S = 251, in-kernel draws (seed 3): finite,
+    non-negative, mass ~1, every chosen rung in [0, S]."""
+    x = W.correlated_3d(1000000, 3)
+    res = _gpu_estimate(pk, x, C3_RANGES, 3)
+    say(f"[C3] gpu {res.ms_total:.1f} ms (ladder {res.ms_ladder:.1f} ms)")
+    mass = W.check_properties(res.density, res.chosen_rung, res.n_rungs,
+                              pk.spacings(pk.initialize_grid(C3_RANGES)), mass_tol=0.2)
+    say(f"  C3 mass {mass:.6f}; chosen rungs {res.chosen_rung.min()}..{res.chosen_rung.max()}")
+
+
+@pytest.mark.timeout(900)
+def test_c3_full_size_short_ladder_matches_oracle(pk):
+    """The same 128^3, n = 1e6, B = 100 workload on a 5-rung ladder (n_steps
+    = 4, so t_k = k * bw_final / 4 reaches bw_final in four steps): every
+    kernel of the 3-D path at its full size (binning of 1e8 draws, 201
+    forward transforms, two rung chunks through both inverse axis passes, the
+    row kernel, convergence, means), whole, against the oracle on the same
+    draws.  The default 251-rung ladder has t < h for its first 90 % of rungs
+    (bw_final ~ 2.6 h here), where sparse cells' log-VMR is set by rounding,
+    and is too long for the oracle in a test; the per-rung log-VMR table is
+    compared on the rungs with t >= h."""
+    x = W.correlated_3d(1000000, 3)
+    seed, steps = 3, 4
+    grid = pk.initialize_grid(C3_RANGES)
+    plan = pk._Plan(grid, pk._params(3, seed=seed, n_steps=steps))
+    try:
+        res = plan.estimate(np.ascontiguousarray(x))
+        vmr = plan.debug_vmr(steps + 1)
+    finally:
+        plan.close()
+    t = time.time()
+    og = O.make_grid(*_spec(C3_RANGES))
+    ref = O.gradepro(og, x, seed=seed, n_steps=steps, threads=THREADS, tables=True)
+    assert ref.rc == 0
+    say(f"[C3 short] gpu {res.ms_total:.1f} ms; oracle {time.time() - t:.1f} s")
+    assert np.allclose(res.bw_final, [ref.info.bw_final[i] for i in range(3)], rtol=1e-12)
+    assert (res.steps_low, res.steps_over) == (ref.info.steps_low, ref.info.steps_over)
+    m = W.compare(res.density, res.chosen_rung, ref.density, ref.chosen, steps + 1)
+    # the oracle with the fixed-point deposits and complex-to-real inverse
+    # transforms the GPU path uses: what is left is the transforms' rounding
+    O.set_deposit_order(2)
+    O.lib().og_set_real_output(1)
+    try:
+        ref_fx = O.gradepro(og, x, seed=seed, n_steps=steps, threads=THREADS)
+    finally:
+        O.set_deposit_order(0)
+        O.lib().og_set_real_output(0)
+    mfx = W.compare(res.density, res.chosen_rung, ref_fx.density, ref_fx.chosen, steps + 1)
+    say(f"  C3 (5 rungs) vs the oracle with fixed-point deposits and real-output inverse "
+        f"transforms: rel L1 {mfx['rel_l1']:.3e} chosen agree {mfx['agree_all']:.5f} "
+        f"(sig {mfx['agree_sig']:.5f})")
+    _check_against_oracle("C3 128^3 (5 rungs)", m,
+                          lambda: _envelope_agreement(og, x, seed, steps + 1, 100, steps))
+    W.check_properties(res.density, res.chosen_rung, steps + 1, pk.spacings(grid), mass_tol=0.2)
+    h = 8.0 / 127.0
+    tk = min(res.bw_final) / steps * np.arange(steps + 1)
+    sig = ref.means >= 1e-2 * ref.means.max(axis=1, keepdims=True)
+    sig[tk < h] = False
+    fin_g, fin_o = np.isfinite(vmr), np.isfinite(ref.vmr)
+    bad = sig & (fin_g != fin_o)
+    both = sig & fin_g & fin_o
+    d = np.abs(vmr - ref.vmr)[both]
+    say(f"  C3 log-VMR on rungs with t >= h: max |d| {d.max():.3e} over {both.sum()} values; "
+        f"finiteness differs at {bad.sum()}")
+    assert bad.sum() <= 1e-3 * sig.sum()
+    assert d.max() < 1e-7
+
+
+# ---------------------------------------------------------------------- C4
+@pytest.mark.timeout(900)
+def test_c4_batch_matches_oracle(pk):
+    """The 64-dataset mix in one batched submission (each request with its own
+    seed): the 1-D and 2-D items against the oracle on the same draws, the
+    64^3 items for their properties and against the same request estimated
+    alone through the plain path."""
+    items = W.batch_mix_items()
+    grids = {}
+    req = []
+    for x, ranges, seed in items:
+        g = grids.setdefault(tuple(ranges), pk.initialize_grid(ranges))
+        req.append((x, g, dict(seed=seed)))
+    t = time.time()
+    try:
+        outs = pk.estimate_batch(req, with_chosen=True)
+    finally:
+        pk.release_batch_memory()
+    say(f"[C4] batch of {len(req)}: {time.time() - t:.2f} s")
+    worst = 0.0
+    for j, ((x, ranges, seed), (dens, chosen)) in enumerate(zip(items, outs)):
+        d = len(ranges)
+        h = pk.spacings(grids[tuple(ranges)])
+        mass = W.check_properties(dens, chosen, 251, h, mass_tol=0.2)
+        if d == 3:
+            if j % 4 == 0:  # a share of the 64^3 items against the plain path
+                alone = _gpu_estimate(pk, x, ranges, seed)
+                assert np.array_equal(alone.density, dens)
+                assert np.array_equal(alone.chosen_rung, chosen)
+            continue
+        og = O.make_grid(*_spec(ranges))
+        ref = O.gradepro(og, x, seed=seed, threads=THREADS)
+        assert ref.rc == 0
+        m = W.compare(dens, chosen, ref.density, ref.chosen, 251)
+        worst = max(worst, m["rel_l1"])
+        _check_against_oracle(f"C4 item {j} ({d}-D, n={x.shape[0]}, mass {mass:.4f})", m,
+                              lambda: _envelope_agreement(og, x, seed, 251, 100, 250),
+                              check_all=False)
+    say(f"[C4] worst rel L1 over the 1-D/2-D items {worst:.3e}")
+
+
+# ---------------------------------------------------------------------- C5
+@pytest.mark.timeout(900)
+def test_c5_stream_alone_equals_mix(pk):
+    """Requests of the served stream (dimension 1-3, n up to 1e6, grids up to
+    1024 / 256^2 / 128^3): each result is bit-identical alone and in the mix
+    of requests in flight together, each has the density properties, and the
+    1-D requests match the oracle."""
+    reqs = W.stream_requests(12)
+    grids = [pk.initialize_grid(r) for _, r, _ in reqs]
+    srv = pk.Server(max_inflight=4)
+    try:
+        alone = []
+        for (x, _, seed), g in zip(reqs, grids):
+            alone.append(srv.wait(srv.submit(x, g, seed=seed)))
+        tickets = [srv.submit(x, g, seed=seed) for (x, _, seed), g in zip(reqs, grids)]
+        mixed = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    say(f"[C5] server stats {st}")
+    for k, ((x, ranges, seed), g, (da, ca), (dm, cm)) in enumerate(zip(reqs, grids, alone, mixed)):
+        assert np.array_equal(da, dm) and np.array_equal(ca, cm), k
+        mass = W.check_properties(da, ca, 251, pk.spacings(g), mass_tol=0.2)
+        say(f"  request {k}: {len(ranges)}-D {g.shape} n={x.shape[0]} mass {mass:.4f}")
+        if len(ranges) == 1:
+            og = O.make_grid(*_spec(ranges))
+            ref = O.gradepro(og, x, seed=seed, threads=THREADS)
+            m = W.compare(da, ca, ref.density, ref.chosen, 251)
+            _check_against_oracle(f"C5 request {k}", m,
+                                  lambda: _envelope_agreement(og, x, seed, 251, 100, 250),
+                                  check_all=False)
+
+
+# ------------------------------------------- the reference's own GPU test
+@pytest.mark.parametrize("n_dims", [1, 2])
+def test_reference_parallel_estimation_mise(pk, n_dims):
+    """gradepro_tests.jl:513-540 (the GPU testset; :485-511 is the same test
+    on the CPU methods): 10,000 N(0, I) samples, 300 points per axis over
+    [-6, 6], default estimator, MISE = sum((f - pdf)^2) * prod(h) / G < 5e-5
+    (test_utils.jl calculate_mise)."""
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(513 + n_dims)
+    x = rng.standard_normal((10000, n_dims))
+    est = pk.initialize_estimation(x, grid=True, grid_ranges=[(-6.0, 6.0, 300)] * n_dims)
+    pk.estimate_density(est, "gradepro")
+    dens = pk.get_density(est)
+    coords = est.grid.get_coordinates()  # N x 300 x ... (Grids.jl:157-169)
+    pts = np.moveaxis(coords, 0, -1).reshape(-1, n_dims)
+    truth = multivariate_normal(np.zeros(n_dims), np.eye(n_dims)).pdf(pts).reshape(dens.shape)
+    dx = float(np.prod(pk.spacings(est.grid)))
+    mise = float(((dens - truth) ** 2).sum() * dx / dens.size)
+    say(f"  {n_dims}-D 300-point grid: MISE {mise:.3e}")
+    assert np.all(np.isfinite(dens))
+    assert mise < 5e-5

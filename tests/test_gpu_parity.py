@@ -123,6 +123,30 @@ def test_binning_cell_edges_bit_exact(L, pk, n_dims):
     assert np.all(es[s > 0] > 0)
 
 
+@pytest.mark.parametrize("n_dims", [1, 2, 3])
+def test_binning_fixed_point_bit_exact(L, pk, n_dims):
+    """The fixed-point binning (the default) equals the oracle's fixed-point
+    deposit mode bit for bit: the same cell index and remainder per sample
+    (floor and rem in FP64), the same llrint quantum per deposit, exact
+    integer sums, the same conversion back."""
+    say(f"[binning fixed point] N={n_dims}")
+    rng = np.random.default_rng(60 + n_dims)
+    shape = [96, 40, 24][:n_dims]
+    g = O.make_grid([-3.0] * n_dims, [3.0] * n_dims, shape)
+    n = 20000
+    x = np.clip(rng.standard_normal((n, n_dims)) * np.array([0.8, 1.1, 0.7][:n_dims]), -2.9, 2.8)
+    idx = O.bootstrap_indices(77, n, 5)
+    rc, s, q = _gpu_dirac(L, pk, g, x, idx, 1)
+    assert rc == 0
+    O.set_deposit_order(2)
+    try:
+        _, es, eq = O.dirac_sequences(g, x, idx)
+    finally:
+        O.set_deposit_order(0)
+    assert np.array_equal(s, es)
+    assert np.array_equal(q, eq)
+
+
 @pytest.mark.parametrize("n_dims", [1, 2])
 def test_binning_multiplicity_overflow(L, pk, n_dims):
     """Fixed-point binning counts how often each sample is drawn per

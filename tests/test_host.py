@@ -159,3 +159,63 @@ def test_get_density_normalisation():
         pk.get_density(est3, normalize=True)
     est3.kde.density[:] = 1.0
     assert np.isclose(pk.get_density(est3, normalize=True, dx=0.5).sum(), 2.0)
+
+
+JULIA_GLUE = os.path.join(ROOT, "parallelkde.jl_amd", "julia", "ROCmBackend.jl")
+
+
+def test_julia_glue_struct_declarations_are_definable():
+    """Julia cannot compute with a type variable in a supertype (`... <:
+    Abstract{N, T, N + 1}` throws at definition time), so every struct of the
+    glue names each supertype parameter directly (GradeProEstimator.jl:673
+    carries the bootstrapped rank M as a parameter of its own)."""
+    text = open(JULIA_GLUE).read()
+    decls = re.findall(r"^(?:mutable\s+)?struct\s+([^\n]+)$", text, flags=re.M)
+    assert decls
+    for d in decls:
+        parts = re.match(r"^(\w+)(?:\{([^}]*)\})?\s+<:\s+(.*)$", d.strip())
+        if not parts:
+            continue
+        own, sup = parts.group(2), parts.group(3)
+        params = {p.split("<:")[0].strip() for p in own.split(",")} if own else set()
+        m = re.search(r"\{([^}]*)\}\s*$", sup.strip())
+        if not m:
+            continue
+        for arg in (a.strip() for a in m.group(1).split(",")):
+            assert re.fullmatch(r"[A-Za-z_][A-Za-z0-9_.]*", arg), (d, arg)
+            if len(arg) <= 2 and arg[0].isupper():
+                assert arg in params, (d, arg)
+
+
+def test_julia_glue_falls_back_to_cpu():
+    """The :rocm constructors warn and fall back to :cpu without a functional
+    device, as ParallelKDE.jl:120-128, Grids.jl:41-44 and KDEs.jl:68-71 do for
+    :cuda."""
+    text = open(JULIA_GLUE).read()
+    for fn in ("Grids.initialize_grid(::IsROCm", "KDEs.initialize_kde(::IsROCm"):
+        i = text.index(fn)
+        body = text[i:text.index("\nend", i)]
+        assert "!functional()" in body and "warn_fallback()" in body and "IsCPU()" in body, fn
+    assert "Falling back to ':cpu'" in text
+
+
+def test_julia_params_mirror_c_struct():
+    """PkdeParams in the glue lists the fields of pkde_gradepro_params in the
+    header's order with the same widths."""
+    text = open(JULIA_GLUE).read()
+    jl = re.search(r"struct PkdeParams\n(.*?)\nend", text, flags=re.S).group(1)
+    jfields = [ln.split("::")[0].strip() for ln in jl.strip().splitlines()]
+    h = open(HEADER).read()
+    cs = re.search(r"typedef struct \{(.*?)\} pkde_gradepro_params;", h, flags=re.S).group(1)
+    cs = re.sub(r"/\*.*?\*/", "", cs, flags=re.S)
+    cfields = [re.sub(r"\[.*\]", "", ln.split()[-1]).rstrip(";") for ln in cs.strip().splitlines()
+               if ln.strip()]
+    assert jfields == cfields
+
+
+def test_integration_notes_state_the_header_abi():
+    h = open(HEADER).read()
+    abi = int(re.search(r"#define PKDE_ABI_VERSION (\d+)", h).group(1))
+    notes = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    stated = {int(v) for v in re.findall(r"ABI version[^0-9]{0,40}?(\d+)", notes)}
+    assert stated == {abi}, stated

@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+import gpu_common as W
 import oracle_ctypes as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -97,6 +98,66 @@ def test_reversed_deposit_order_is_the_same_binning():
     assert np.allclose(s1, s0, rtol=1e-13, atol=0) and np.allclose(q1, q0, rtol=1e-13, atol=0)
     assert np.array_equal(s1 > 0, s0 > 0)
     assert not np.array_equal(s1, s0)  # a different rounding somewhere
+
+
+def test_fixed_point_deposits_round_the_same_sum():
+    """Deposit mode 2 (the fixed-point deterministic mode) rounds each deposit
+    to 2^-fb of a whole draw (fb = 61 - bits of n) and sums exactly: each
+    node is within (deposits at the node) half-quanta of the FP64 sum, the
+    support is the same, and any order of the draws gives the same bits."""
+    rng = np.random.default_rng(12)
+    g = O.make_grid([-3.0, -3.0, -2.0], [3.0, 3.0, 2.0], [17, 13, 11])
+    n = 3000
+    x = rng.uniform(-2.9, 2.9, size=(n, 3)) * np.array([1.0, 1.0, 0.6])
+    idx = rng.integers(0, n, size=(3, n)).astype(np.int32)
+    _, s0, q0 = O.dirac_sequences(g, x, idx)
+    fb = O.lib().og_fixed_frac_bits(n)
+    assert fb == 61 - int(n).bit_length()
+    O.set_deposit_order(2)
+    try:
+        _, s2, q2 = O.dirac_sequences(g, x, idx)
+        _, s2p, q2p = O.dirac_sequences(g, x, idx[:, ::-1].copy())
+    finally:
+        O.set_deposit_order(0)
+    hp = np.prod([g.h[i] for i in range(3)])
+    qM = 2.0 ** -fb / (n * hp)
+    qV = 2.0 ** -fb / (n * hp) ** 2
+    assert np.abs(s2 - s0).max() <= 0.5 * qM * n * 8 and np.abs(q2 - q0).max() <= 0.5 * qV * n * 8
+    assert np.array_equal(s2 > 0, s0 > 0)
+    assert np.array_equal(s2, s2p) and np.array_equal(q2, q2p)
+    # a single deposit is exactly llrint(term * scale) / scale
+    one = np.array([[0.123, -0.456, 0.0789]])
+    _, s1, _ = O.dirac_sequences(g, one, np.zeros((1, 1), dtype=np.int32))
+    O.set_deposit_order(2)
+    try:
+        _, s1f, _ = O.dirac_sequences(g, one, np.zeros((1, 1), dtype=np.int32))
+    finally:
+        O.set_deposit_order(0)
+    scale = 2.0 ** O.lib().og_fixed_frac_bits(1) * 1 * hp
+    nz = s1 > 0
+    assert np.array_equal(s1f[nz], np.rint(s1[nz] * scale) * (1.0 / scale))
+
+
+def test_real_output_inverse_is_the_same_estimate():
+    """The real-output probe (og_set_real_output) drops only the imaginary
+    rounding noise of the inverse transforms: the log-VMR tables agree to
+    rounding where the means are significant (rungs past the first three,
+    where empty cells of a resample still hold nothing but that noise), and
+    the estimate moves only by the few rungs those first rungs decide."""
+    x = W.gaussian_1d(10000, 3)
+    g = O.make_grid([-5.0], [5.0], [256])
+    idx = O.bootstrap_indices(4, x.shape[0], 20)
+    r0 = O.gradepro(g, x, idx_bs=idx, n_bootstraps=20, n_steps=60, tables=True)
+    O.lib().og_set_real_output(1)
+    try:
+        r1 = O.gradepro(g, x, idx_bs=idx, n_bootstraps=20, n_steps=60, tables=True)
+    finally:
+        O.lib().og_set_real_output(0)
+    sig = r0.means >= 1e-2 * r0.means.max(axis=1, keepdims=True)
+    sig[:3] = False
+    assert np.nanmax(np.abs(r1.vmr - r0.vmr)[sig]) < 1e-8
+    assert np.abs(r1.density - r0.density).sum() / r0.density.sum() < 1e-4
+    assert np.mean(r1.chosen == r0.chosen) >= 0.98
 
 
 def test_fft_round_trip_and_parseval():
