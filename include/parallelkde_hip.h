@@ -231,6 +231,11 @@ typedef struct {
 
 int pkde_gradepro_estimate_batch(pkde_ctx *ctx, int32_t count, const pkde_request *reqs,
                                  int32_t *status_out);
+/* The same on device-resident data: each request's samples, density_out
+ * and chosen_out (nullable) are device pointers; nothing but the status
+ * flags crosses PCIe.  Returns when every request has finished. */
+int pkde_gradepro_estimate_batch_device(pkde_ctx *ctx, int32_t count, const pkde_request *reqs,
+                                        int32_t *status_out);
 /* Frees what the batched form keeps resident in the context (its streams'
  * plans and chunk buffers); the next batch allocates them again. */
 int pkde_context_release_batch(pkde_ctx *ctx);

@@ -142,8 +142,11 @@ int grow_chunk(void*& Y, uint64_t& Y_bytes, uint64_t need, const char* what) {
 }  // namespace
 
 // ------------------------------------------------------------------- batch
-extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const pkde_request* reqs,
-                                            int32_t* status_out) {
+// device_io: samples, density_out and chosen_out are device pointers (the
+// samples already in device memory, the outputs written there); otherwise
+// host pointers staged through pinned memory.
+static int estimate_batch_impl(pkde_ctx* ctx, int32_t count, const pkde_request* reqs,
+                               int32_t* status_out, bool device_io) {
   if (!ctx || count < 0 || (count > 0 && !reqs)) return fail(PKDE_INVALID_ARGUMENT, "invalid argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
@@ -176,8 +179,8 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
   };
   for (int i = 0; i < count; ++i) {
     const pkde_request& r = reqs[i];
-    const int64_t G = st[i] == PKDE_OK ? fp[i].G : 0;
-    const int64_t n = st[i] == PKDE_OK ? r.n : 0;
+    const int64_t G = st[i] == PKDE_OK && !device_io ? fp[i].G : 0;
+    const int64_t n = st[i] == PKDE_OK && !device_io ? r.n : 0;
     off_x[i] = take((size_t)n * r.ndims * 8);
     off_d[i] = take((size_t)G * 8);
     off_c[i] = take((size_t)G * 2);
@@ -275,6 +278,14 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
     pkde_plan* pl = used[i];
     hipStream_t s = ctx->lanes[lane_of[i]].stream;
     pl->p.rng_seed = r.params.rng_seed;
+    if (device_io) {
+      st[i] = plan_enqueue(pl, r.samples, r.n, nullptr, r.density_out,
+                           r.chosen_out ? r.chosen_out : pl->d.chosen, s, false);
+      if (!st[i])
+        st[i] = check_hip(hipMemcpyAsync(ctx->stage + off_e[i], pl->d.err, 4, hipMemcpyDeviceToHost, s),
+                          "flag");
+      continue;
+    }
     memcpy(ctx->stage + off_x[i], r.samples, (size_t)r.n * r.ndims * 8);
     st[i] = check_hip(hipMemcpyAsync(pl->d.x, ctx->stage + off_x[i], (size_t)r.n * r.ndims * 8,
                                      hipMemcpyHostToDevice, s), "upload");
@@ -301,7 +312,7 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
       memcpy(&err, ctx->stage + off_e[i], 4);
       if (err & 1)
         st[i] = fail(PKDE_OUT_OF_GRID, "Data points lie beyond grid.");
-      else {
+      else if (!device_io) {
         memcpy(reqs[i].density_out, ctx->stage + off_d[i], (size_t)G * 8);
         if (reqs[i].chosen_out) memcpy(reqs[i].chosen_out, ctx->stage + off_c[i], (size_t)G * 2);
       }
@@ -310,6 +321,16 @@ extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const 
     if (st[i] != PKDE_OK && first_rc == PKDE_OK) first_rc = st[i];
   }
   return rc ? rc : first_rc;
+}
+
+extern "C" int pkde_gradepro_estimate_batch(pkde_ctx* ctx, int32_t count, const pkde_request* reqs,
+                                            int32_t* status_out) {
+  return estimate_batch_impl(ctx, count, reqs, status_out, false);
+}
+
+extern "C" int pkde_gradepro_estimate_batch_device(pkde_ctx* ctx, int32_t count,
+                                                   const pkde_request* reqs, int32_t* status_out) {
+  return estimate_batch_impl(ctx, count, reqs, status_out, true);
 }
 
 extern "C" int pkde_gradepro_request_footprint(pkde_ctx* ctx, const pkde_request* req,
