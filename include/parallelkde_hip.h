@@ -246,8 +246,10 @@ int pkde_context_release_batch(pkde_ctx *ctx);
 int pkde_gradepro_request_footprint(pkde_ctx *ctx, const pkde_request *req, uint64_t *bytes_out);
 
 /* ---- serving form: continuous batching with HBM-footprint admission ----
- * A server thread admits requests, first come first served, as soon as a
- * stream slot is free and the device memory the request still needs (a plan
+ * A server thread admits requests as soon as a stream slot is free --
+ * among the first 16 queued, the one with the least estimated device work
+ * first, but none passed over more than 16 times -- and the device memory
+ * the request still needs (a plan
  * for its shape, a larger chunk buffer for the slot) fits the budget next to
  * everything resident (bytes_in_use: the pooled plans and the slots' chunk
  * buffers); idle plans are evicted, least recently used first, to make room.

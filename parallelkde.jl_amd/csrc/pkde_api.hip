@@ -128,14 +128,15 @@ static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; 
 static bool use_slab3(const pkde_plan* pl) { return pl->prec && slab3_ok(pl->g); }
 static int y_units(const pkde_plan* pl) { return pl->g.N == 3 && !use_slab3(pl) ? 2 : 1; }
 
-// Rungs per chunk: as many as the chunk budget holds (8 GiB, or 1/32 of the
-// device memory if that is less), and -- unless the budget was set
+// Rungs per chunk: as many as the chunk budget holds (16 GiB, or 1/16 of
+// the device memory if that is less), and -- unless the budget was set
 // explicitly -- at least 4 when they fit in 1/8 of the device memory; then
 // spread evenly over the chunks the ladder needs, so the last chunk is not a
 // small remainder.  Every chunk pays the tails of its kernels (the waves of
 // its last, partly filled round) and reads the spectra once: at 256^2 the
-// ladder takes 11.97 ms with 4 chunks against 12.38 ms with 7, at 64^3
-// 66.5 ms with 13 against 74.8 ms with 26.
+// ladder takes 11.97 ms with 4 chunks against 12.38 ms with 7; at 64^3
+// 45.9 ms with 7 chunks (16 GiB) against 48.4 ms with 14 (8 GiB), and no
+// less with 32 GiB.
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
@@ -143,16 +144,11 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   if (per_rung == 0) return (uint64_t)fsize(pl) * 2 * rs * 2;  // :rot: the means pass only
   uint64_t budget = pl->chunk_budget ? pl->chunk_budget : pl->ctx->chunk_bytes;
   if (!pl->ctx->chunk_fixed && pl->ctx->mem_total)
-    budget = std::min<uint64_t>(budget, pl->ctx->mem_total / 32);
+    budget = std::min<uint64_t>(budget, pl->ctx->mem_total / 16);
   uint64_t rungs = std::max<uint64_t>(1, budget / units / per_rung);
-  // at least 4 rungs when they fit in 1/8 of the device memory (2 rungs in
-  // 1/16 for a lane or slot plan, several of which share the device)
-  if (!pl->ctx->chunk_fixed) {
-    const bool lane = pl->chunk_budget != 0;
-    const uint64_t floor_r = std::min<uint64_t>(lane ? 2 : 4, pl->ctx->mem_total / (lane ? 16 : 8) /
-                                                                   units / per_rung);
-    rungs = std::max<uint64_t>(rungs, floor_r);
-  }
+  // at least 4 rungs when they fit in 1/8 of the device memory
+  if (!pl->ctx->chunk_fixed)
+    rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
   const uint64_t S = (uint64_t)pl->S;
   rungs = std::min<uint64_t>(rungs, S);
   const uint64_t chunks = (S + rungs - 1) / rungs;

@@ -43,11 +43,12 @@ struct pkde_ctx {
   double* tw64 = nullptr;  // exp(-2 pi i k / L) for L = 2..4096 at offset L-2
   float* tw32 = nullptr;
   std::mutex mu;
-  uint64_t chunk_bytes = 8ull << 30;  // per-plan budget of the rung-chunk buffer
+  uint64_t chunk_bytes = 16ull << 30;  // per-plan budget of the rung-chunk buffer
   bool chunk_fixed = false;           // budget set by PKDE_CHUNK_BYTES
   // budget of one stream lane's (batched form) or server slot's shared
-  // chunk buffer
-  uint64_t lane_chunk_bytes = 4ull << 30;
+  // chunk buffer: the plain plan's, so a request runs the same chunks in
+  // every form
+  uint64_t lane_chunk_bytes = 16ull << 30;
   uint64_t mem_total = 0;             // device memory, bytes
   // batched form
   std::vector<pkde_batch_lane> lanes;

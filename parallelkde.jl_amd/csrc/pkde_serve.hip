@@ -12,8 +12,10 @@
 // Serving: a worker thread runs continuous batching over `max_inflight`
 // device streams (slots).  A request's device footprint is computed on the
 // host when it is submitted; a request whose footprint alone exceeds the
-// memory budget is rejected with PKDE_OUT_OF_MEMORY and the others go on.  A
-// request is admitted as soon as a slot is free and the memory it still
+// memory budget is rejected with PKDE_OUT_OF_MEMORY and the others go on.
+// Short requests go first (least estimated work among the first queued ones,
+// with a bound on how often one request can be passed over).  A request is
+// admitted as soon as a slot is free and the memory it still
 // needs (a new plan for its shape, a larger chunk buffer for the slot) fits
 // the budget next to everything resident; idle plans are evicted, least
 // recently used first, to make room.  Plans live in a pool keyed by shape:
@@ -355,7 +357,15 @@ struct Job {
   double t_submit = 0, t_done = 0;
   Footprint fp;
   int64_t n_cap = 0;  // samples the plan is reserved for (a power of two)
+  int bypassed = 0;   // times a later, smaller request was admitted first
 };
+
+// Admission order: among the first kLookahead queued requests, the one with
+// the least estimated device work goes first (short requests do not wait
+// behind long ones), unless a request ahead of it has already been passed
+// over kMaxBypass times -- then that one goes, so no request starves.
+static constexpr int kLookahead = 16;
+static constexpr int kMaxBypass = 16;
 
 struct PoolPlan {
   pkde_plan* pl = nullptr;
@@ -554,19 +564,34 @@ static void server_loop(pkde_server* sv) {
         finish_job(sv, sl);
         progressed = true;
       }
-    // admissions (continuous batching), first come first served
+    // admissions (continuous batching), shortest estimated work first
+    // within the look-ahead window, with bounded bypass
     for (auto& sl : sv->slots) {
       if (sl.job) continue;
       std::shared_ptr<Job> job;
+      size_t pick = 0;
       {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->queue.empty()) break;
-        job = sv->queue.front();
+        const size_t win = std::min<size_t>(sv->queue.size(), kLookahead);
+        for (size_t i = 0; i < win; ++i) {
+          if (sv->queue[i]->bypassed >= kMaxBypass) {
+            pick = i;
+            break;
+          }
+          if (sv->queue[i]->fp.work < sv->queue[pick]->fp.work) pick = i;
+        }
+        job = sv->queue[pick];
       }
+      auto dequeue = [&] {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (size_t i = 0; i < pick; ++i) sv->queue[i]->bypassed++;
+        sv->queue.erase(sv->queue.begin() + pick);
+      };
       if (job->fp.plan + job->fp.chunk > sv->cfg.memory_budget) {
         {
           std::lock_guard<std::mutex> lk(sv->mu);
-          sv->queue.pop_front();
+          sv->queue.erase(sv->queue.begin() + pick);
         }
         complete(sv, job, PKDE_OUT_OF_MEMORY, true);
         progressed = true;
@@ -578,10 +603,7 @@ static void server_loop(pkde_server* sv) {
         for (auto& o : sv->slots) busy = busy || (bool)o.job;
         if (busy) break;  // wait until running requests release memory
       }
-      {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->queue.pop_front();
-      }
+      dequeue();
       if (r <= 0) complete(sv, job, r < 0 ? -r : PKDE_OUT_OF_MEMORY, r == 0);
       progressed = true;
     }
