@@ -150,6 +150,25 @@ int pkde_gradepro_estimate(pkde_plan *plan, const double *samples, int64_t n,
 int pkde_gradepro_estimate_device(pkde_plan *plan, const double *d_samples, int64_t n,
                                   const int32_t *d_bootstrap_idx, double *d_density,
                                   int16_t *d_chosen_rung, void *stream);
+/* Allocation-free estimate.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same estimate
+ * as pkde_gradepro_estimate_device on a plan with this grid and keywords, on
+ * device-resident samples, asynchronous on `stream` (NULL: the context
+ * stream); every working buffer comes from a file-scope static device array
+ * of 32 MiB per GPU, held until the process ends.  It calls no hipMalloc,
+ * hipFree or allocating library routine.  Limits: 1-D power-of-two grids of
+ * 16 to 1024 points (grid lo, hi, spacing h), 2 to 131072 samples, an
+ * explicit n_steps, FP64 with fixed-point binning, and a workspace that fits
+ * the array (B = 100 and S = 251 take about 26 MiB at 1024 points); others
+ * return PKDE_UNSUPPORTED / PKDE_INVALID_ARGUMENT / PKDE_OUT_OF_MEMORY.
+ * Calls share their GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller; each
+ * call clears what it reads before writing it.  d_chosen may be NULL. */
+int pkde_gradepro_estimate_1d_noalloc(pkde_ctx *ctx, int64_t n_points, double lo, double hi,
+                                      double h, const pkde_gradepro_params *params,
+                                      const double *d_samples, int64_t n, double *d_density,
+                                      int16_t *d_chosen, void *stream);
 /* Waits for the plan's last device estimate; PKDE_OUT_OF_GRID if it saw a
  * sample outside the grid (check_data, DirectSpace.jl:479-498). */
 int pkde_gradepro_check(pkde_plan *plan);

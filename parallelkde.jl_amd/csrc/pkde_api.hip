@@ -12,6 +12,14 @@
 
 using namespace pkde;
 
+// Device memory of the allocation-free estimate (pkde_gradepro_estimate_1d_
+// noalloc): every buffer of its plan is carved from this array.  Each GPU
+// holds its own copy; host code reaches it only through the address
+// hipGetSymbolAddress returns for the device (pkde_ctx::scratch).
+static constexpr uint64_t kScratchBytes = 32ull << 20;
+static constexpr int64_t kNoallocMaxSamples = 131072;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace pkde {
 
 static thread_local std::string g_last_error;
@@ -977,6 +985,17 @@ int pkde_context_create(int device, pkde_ctx** out) {
     c->chunk_fixed = true;
   }
   {
+    // this device's copy of the static scratch array, with the device current
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) {
+      c->scratch = (unsigned char*)ptr;
+    } else {
+      c->scratch = nullptr;  // the allocation-free entry point then refuses to run
+      (void)hipGetLastError();
+    }
+  }
+  {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->mem_total = tot;
   }
@@ -1391,3 +1410,109 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free estimate
+namespace {
+
+// The host-computed tables of plan creation, computed on the device with the
+// same IEEE operations (so the results are bit-identical): Fourier
+// coordinates 2 pi fftfreq(n, 1/h) (Grids.jl:225-231), Welford weights
+// 1 / (b + 1), the given bandwidth, zeroed per-axis twiddles (power-of-two
+// lengths take the context's table).
+__global__ void k_noalloc_tables(int n1, double h1, double* kfreq, int B, double* invk, int bw_given,
+                                 double bw0, double* bw, double* twg) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < n1) {
+    const double mult = ((1.0 / h1) / (double)n1) * 6.283185307179586;
+    const int nn = (n1 + 1) >> 1;
+    kfreq[tid] = (double)(tid < nn ? tid : tid - n1) * mult;
+    twg[2 * tid] = 0.0;
+    twg[2 * tid + 1] = 0.0;
+  }
+  if (tid < B) invk[tid] = 1.0 / (double)(tid + 1);
+  if (tid == 0 && bw_given) bw[0] = bw0;
+}
+
+struct Carve {
+  unsigned char* base;
+  uint64_t used = 0;
+  template <typename T>
+  T* take(uint64_t bytes) {
+    T* p = (T*)(base + used);
+    used += (bytes + 255) & ~(uint64_t)255;
+    return p;
+  }
+};
+
+}  // namespace
+
+extern "C" int pkde_gradepro_estimate_1d_noalloc(pkde_ctx* ctx, int64_t n_points, double lo,
+                                                 double hi, double h,
+                                                 const pkde_gradepro_params* params,
+                                                 const double* d_samples, int64_t n,
+                                                 double* d_density, int16_t* d_chosen,
+                                                 void* stream) {
+  if (!ctx || !d_samples || !d_density) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (!ctx->scratch)
+    return fail(PKDE_DEVICE_ERROR, "the static scratch array has no address on this device");
+  if (n_points < 16 || n_points > 1024 || (n_points & (n_points - 1)))
+    return fail(PKDE_UNSUPPORTED, "the allocation-free estimate takes 1-D power-of-two grids of 16 to "
+                                  "1024 points");
+  if (n < 2 || n > kNoallocMaxSamples)
+    return fail(PKDE_INVALID_ARGUMENT, "the allocation-free estimate takes 2 to 131072 samples");
+  pkde_gradepro_params p;
+  if (params)
+    p = *params;
+  else
+    pkde_gradepro_params_default(&p);
+  if (p.n_steps <= 0)
+    return fail(PKDE_UNSUPPORTED, "the allocation-free estimate needs n_steps (a bw_step ladder is "
+                                  "sized from the samples)");
+  if (p.precision != 1 || p.deterministic_binning != 1)
+    return fail(PKDE_UNSUPPORTED, "the allocation-free estimate runs the FP64 fixed-point path");
+  pkde_plan pl;
+  int rc = plan_setup_host(ctx, 1, &n_points, &lo, &hi, &h, &p, &pl);
+  if (rc) return rc;
+  const GridDev& g = pl.g;
+  const int B = pl.B, S = pl.S;
+  const int64_t G = g.G, F = 2 * (int64_t)B + 1, fs = fsize(&pl);
+  Carve c{ctx->scratch};
+  PlanBuffers& d = pl.d;
+  d.acc = c.take<void>((uint64_t)F * G * 8);
+  d.specD = c.take<cplx<double>>((uint64_t)F * fs * 16);
+  d.vmr = c.take<double>((uint64_t)S * G * 8);
+  if (G <= kConvTermsMaxG) d.conv = c.take<double>((uint64_t)S * G * 16);
+  d.chosen = c.take<int16_t>((uint64_t)G * 2);
+  d.sel = c.take<int16_t>((uint64_t)G * 2);
+  d.err = c.take<int>(256);
+  d.params = c.take<double>(256);
+  d.mean = c.take<double>(256);
+  d.bw = c.take<double>(256);
+  d.partial = c.take<double>((uint64_t)kMomentBlocks * 2 * PKDE_MAX_DIMS * 8);
+  d.times = c.take<double>((uint64_t)S * PKDE_MAX_DIMS * 8);
+  d.dlogt = c.take<double>((uint64_t)S * 8);
+  d.vscale = c.take<double>((uint64_t)S * 8);
+  d.cV = c.take<double>((uint64_t)S * 8);
+  d.gM = c.take<void>((uint64_t)S * g.ksum * 8);
+  d.gV = c.take<void>((uint64_t)S * g.ksum * 8);
+  d.invk = c.take<double>((uint64_t)B * 8);
+  d.kfreq = c.take<double>((uint64_t)g.ksum * 8);
+  d.twg64 = c.take<double>((uint64_t)g.ksum * 16);
+  d.binwork_bytes = binsort_bytes(g, n, B);
+  d.binwork = c.take<void>(d.binwork_bytes);
+  d.S_alloc = S;
+  if (c.used > kScratchBytes)
+    return fail(PKDE_OUT_OF_MEMORY, "the estimate needs " + std::to_string(c.used >> 20) +
+                                        " MiB of scratch; the static array holds " +
+                                        std::to_string(kScratchBytes >> 20) + " MiB");
+  rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  (void)hipGetLastError();
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  const int thr = 256;
+  const int blocks = (int)((std::max<int64_t>(g.ksum, B) + thr - 1) / thr);
+  k_noalloc_tables<<<blocks, thr, 0, s>>>(g.n[0], g.h[0], d.kfreq, B, d.invk, pl.bw_given ? 1 : 0,
+                                          pl.bw_host[0], d.bw, d.twg64);
+  return enqueue_typed<double>(&pl, d_samples, n, nullptr, d_density,
+                               d_chosen ? d_chosen : d.chosen, s, false);
+}

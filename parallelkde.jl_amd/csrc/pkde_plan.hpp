@@ -66,6 +66,9 @@ struct pkde_ctx {
   std::mutex tune_mu;
   std::vector<std::pair<std::vector<int64_t>, pkde::TuneShape>> tune;
   std::atomic<int> tune_gen{0};
+  // this device's copy of the library's static scratch array (the
+  // allocation-free entry point), resolved once when the context is created
+  unsigned char* scratch = nullptr;
 };
 
 namespace pkde {

@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = [
     "pkde_gradepro_debug_enable", "pkde_gradepro_debug_vmr", "pkde_gradepro_convergence",
     "pkde_gradepro_graph_enable", "pkde_gradepro_autotune",
     "pkde_dirac_sequences", "pkde_gradepro_estimate_batch", "pkde_context_release_batch",
-    "pkde_gradepro_estimate_batch_device",
+    "pkde_gradepro_estimate_batch_device", "pkde_gradepro_estimate_1d_noalloc",
     "pkde_gradepro_request_footprint",
     "pkde_server_create", "pkde_server_submit", "pkde_server_poll", "pkde_server_wait",
     "pkde_server_stats_get", "pkde_server_destroy",
@@ -136,6 +136,9 @@ def load() -> C.CDLL:
                                        C.c_int, vp, vp]
     L.pkde_gradepro_estimate_batch.argtypes = [vp, C.c_int32, C.POINTER(Request), vp]
     L.pkde_context_release_batch.argtypes = [vp]
+    L.pkde_gradepro_estimate_1d_noalloc.argtypes = [vp, C.c_int64, C.c_double, C.c_double,
+                                                    C.c_double, C.POINTER(Params), vp, C.c_int64,
+                                                    vp, vp, vp]
     L.pkde_gradepro_estimate_batch_device.argtypes = [vp, C.c_int32, C.POINTER(Request), vp]
     L.pkde_gradepro_request_footprint.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.pkde_server_create.argtypes = [vp, C.POINTER(ServerCfg), C.POINTER(vp)]
