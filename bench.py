@@ -313,7 +313,8 @@ def run_batch(args):
         f"{len(by_dim[3])} 3-D); warmup")
     for _ in range(max(1, args.warmup)):
         batch(reqs_all, len(items))
-        plain(everything)
+        if not args.batch_only:
+            plain(everything)
     torch.cuda.synchronize()
 
     def timed(fn, reps):
@@ -325,8 +326,15 @@ def run_batch(args):
         return (time.perf_counter() - t0) / reps
 
     t_batch = timed(lambda: batch(reqs_all, len(items)), args.steps)
-    gpu_out = [(d.cpu().numpy(), c.cpu().numpy()) for d, c in zip(dens, chos)]
     log(f"[bench] batch: {1e3 * t_batch:.1f} ms per batch")
+    if args.batch_only:
+        # profiling runs: the batched submissions alone (warmup + steps)
+        for p in plans.values():
+            p.close()
+        pk.release_batch_memory()
+        return {"workload": "batch only", "batches": max(1, args.warmup) + args.steps,
+                "ms_per_batch": 1e3 * t_batch, "estimates_per_s": len(items) / t_batch}, items, None
+    gpu_out = [(d.cpu().numpy(), c.cpu().numpy()) for d, c in zip(dens, chos)]
     t_plain = timed(lambda: plain(everything), args.steps)
     plain(everything, record=True)
     log(f"[bench] plain: {1e3 * t_plain:.1f} ms per 64 requests")
@@ -545,6 +553,8 @@ def main():
     ap.add_argument("--workload", default="batch")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch workload: time the batched submissions only (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=0.0)
@@ -570,7 +580,7 @@ def main():
         return
     if args.workload == "batch":
         out, items, gpu_out = run_batch(args)
-        if not args.no_cpu_baseline:
+        if not args.no_cpu_baseline and not args.batch_only:
             log("[bench] cpu baseline and accuracy (oracle, bounded sample)")
             out["cpu_baseline"], out["accuracy"] = cpu_leg_batch(items, gpu_out, args.cpu_budget)
         print(json.dumps(out), flush=True)
