@@ -409,6 +409,13 @@ struct pkde_server {
   bool stop = false;
 };
 
+// Pooled plans are reserved for at least this many samples, and the slots'
+// pinned buffers are sized for this many grid points and samples up front:
+// growing a device buffer frees the old one, and a free synchronises the
+// whole device, which would stall every slot in flight.
+static constexpr int64_t kServeMinSamples = 1 << 20;
+static constexpr int64_t kServeMinPoints = 1 << 21;
+
 static uint64_t resident_bytes(const pkde_server* sv) {
   uint64_t b = 0;
   for (const auto& p : sv->pool) b += p->bytes;
@@ -509,7 +516,15 @@ static int start_job(pkde_server* sv, Slot& sl, const std::shared_ptr<Job>& job)
     need += job->fp.plan > pp->bytes ? job->fp.plan - pp->bytes : 0;
   if (!make_room(sv, need, pp)) return 0;
   int st = PKDE_OK;
-  if ((st = grow_chunk(sl.Y, sl.Y_bytes, job->fp.chunk, "slot chunk buffer"))) return -st;
+  if (sl.Y_bytes < job->fp.chunk) {
+    // grow to the largest chunk buffer any slot holds, so that the slots
+    // reach their final size in few growths
+    uint64_t want = job->fp.chunk;
+    for (const auto& o : sv->slots) want = std::max(want, o.Y_bytes);
+    if (!make_room(sv, want - sl.Y_bytes + (need - (job->fp.chunk - sl.Y_bytes)), pp))
+      want = job->fp.chunk;
+    if ((st = grow_chunk(sl.Y, sl.Y_bytes, want, "slot chunk buffer"))) return -st;
+  }
   if (!pp) {
     auto np = std::make_unique<PoolPlan>();
     pkde_plan* pl = nullptr;
@@ -640,6 +655,7 @@ int pkde_server_create(pkde_ctx* ctx, const pkde_server_cfg* cfg, pkde_server** 
     if (!rc) rc = check_hip(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "event");
     if (!rc) rc = check_hip(hipHostMalloc((void**)&sl.err_host, 64, hipHostMallocDefault), "pinned");
+    if (!rc) rc = grow_host(sl, kServeMinPoints, kServeMinSamples * 3);
   }
   if (rc) {
     pkde_server_destroy(sv);
@@ -659,7 +675,7 @@ int pkde_server_submit(pkde_server* sv, const pkde_request* req, uint64_t* ticke
   job->key = key_of(*req);
   // footprint on the host, for a plan reserved for the next power of two of
   // samples (plans of a shape are reused by later requests)
-  job->n_cap = pow2_ceil(req->n);
+  job->n_cap = std::max<int64_t>(pow2_ceil(req->n), kServeMinSamples);
   job->fp = footprint_of(sv->ctx, *req, job->n_cap);
   if (job->fp.status) return job->fp.status;
   job->samples.assign(req->samples, req->samples + (size_t)req->n * req->ndims);
