@@ -131,9 +131,11 @@ static int64_t fsize(const pkde_plan* pl) { return (int64_t)pl->g.H1 * pl->g.Q; 
 
 // The rung-chunk buffer has y_units() equal parts (two in 3-D: the axis-2
 // result and the axis-3 result in the tiled row layout; one where the fused
-// slab kernel reads the axis-2 result directly); each part holds at least one
-// rung of the 2B bootstrap fields.
-static bool use_slab3(const pkde_plan* pl) { return pl->prec && slab3_ok(pl->g); }
+// slab kernel, whole or in quarters of axis 3, reads the axis-2 result
+// directly); each part holds at least one rung of the 2B bootstrap fields.
+static bool use_slab3(const pkde_plan* pl) {
+  return pl->prec && (slab3_ok(pl->g) || slab3_split_ok(pl->g));
+}
 static int y_units(const pkde_plan* pl) { return pl->g.N == 3 && !use_slab3(pl) ? 2 : 1; }
 
 // Rungs per chunk: as many as the chunk budget holds (16 GiB, or 1/16 of
@@ -405,7 +407,10 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
         }
       }
       cplx<T>* Y1 = N == 3 && !slab ? Y2 : Y;
-      const int mode1 = N == 2 ? (rows ? 1 : 0) : (rows3 ? 2 : 0);
+      // 3-D: [k1][j2][k3] for the axis-3 pass and the whole slab kernel; the
+      // first radix-4 step of axis 3 on store for the slab kernel in quarters
+      const int mode1 = N == 2 ? (rows ? 1 : 0)
+                               : (rows3 ? (slab && !slab3_ok(g) ? 3 : 2) : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s,
                                  pl->tune.split)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");

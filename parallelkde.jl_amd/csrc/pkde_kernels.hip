@@ -342,9 +342,12 @@ struct AxisArgs {
   // (o: outer pencil index, e: position along the axis)
   int64_t oO, oE;
   // ladder pass: TJ outer pencils per block, store mode 0 [k1][q], 1 tiled
-  // rows, 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass); pack:
-  // column 0 carries X(k1 = 0) + i X(k1 = H1 columns), H1 = n1 / 2
+  // rows, 2 [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass), 3 the
+  // first radix-4 step of the axis-3 inverse applied on store (see
+  // k_axis_ladder); pack: column 0 carries X(k1 = 0) + i X(k1 = H1 columns),
+  // H1 = n1 / 2
   int TJ, log2TJ, log2otiles, mode, pack;
+  int goff3;  // mode 3: offset of grid axis 3 in the per-rung Gaussian tables
   int nsplit;  // ladder pass: the chunk's rungs are split over nsplit blocks per tile
   const cplx<T>* tw;
   // mixed-radix plan and twiddle table of the axis (GEN kernels)
@@ -610,8 +613,16 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // Gaussian factor, first pass from registers, remaining passes through LDS,
 // store of the block's pencils in layout `mode`: 0 [k1][q], 1 the tiled rows
 // of the row kernel (2-D; the block's tile is one contiguous run), 2
-// [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass).  The arithmetic
-// is that of k_axis_wave with DIR = +1.
+// [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass), 3 the first
+// radix-4 decimation-in-frequency step of the axis-3 inverse transform applied
+// on store: the block's 16 pencils are k3 = k' + s n3/4 for 4 consecutive k'
+// and s = 0..3 (each scaled by its axis-3 factor g3(k3) with the axis-2
+// factors), and it stores, for each j2 and residue class p = j3 mod 4,
+// c_p(k') = w^(p k') sum_s i^(p s) Y(k' + s n3/4), w =
+// exp(2 pi i / n3), at [k1][j2][k' / 4][p][k' mod 4] -- one contiguous
+// 256-byte run per (k1, j2); the n3/4-point inverse transform of c_p over k'
+// gives the outputs j3 = 4 m + p (k_slab3 with SPLIT = 4).  The arithmetic is
+// that of k_axis_wave with DIR = +1.
 template <typename T, int LOG2L>
 __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -648,16 +659,20 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   const bool act = kk < TKe && p < TK * TJ;
   const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
   const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
+  // the pencil's outer index: o0 + jj, or in mode 3 k3 = k' + s n3/4 with
+  // k' = 4 ot + (jj mod 4), s = jj / 4
+  const int64_t oq = a.mode == 3 ? (int64_t)ot * 4 + (jj & 3) + (int64_t)(jj >> 2) * (a.O >> 2)
+                                 : o0 + jj;
   cplx<T> x[E];
   if (act) {
-    const cplx<T>* col = src + (int64_t)(k0 + kk) * a.Q + (o0 + jj) * L;
+    const cplx<T>* col = src + (int64_t)(k0 + kk) * a.Q + oq * L;
 #pragma unroll
     for (int u = 0; u < E / R1; ++u)
 #pragma unroll
       for (int r = 0; r < R1; ++r) x[u * R1 + r] = col[t + u * tpp + r * nb1];
     if (a.pack && k0 + kk == 0) {
       // X(0) + i X(n1/2): the Nyquist column shares column 0's pencil
-      const cplx<T>* colN = src + (int64_t)a.H1 * a.Q + (o0 + jj) * L;
+      const cplx<T>* colN = src + (int64_t)a.H1 * a.Q + oq * L;
 #pragma unroll
       for (int u = 0; u < E / R1; ++u)
 #pragma unroll
@@ -678,15 +693,29 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
     for (int id = tid; id < ((TKw - TKe) << LOG2L); id += kThreads)
       buf[(TKe + (id >> LOG2L)) * PL + pad8(id & (L - 1))] = {(T)0, (T)0};
   }
+  // mode 3: a thread's entries of the store all have the same j = tid mod 16
+  // (class p, k' = 4 ot + c), so its twiddle exp(+2 pi i p k' / n3) is fixed
+  cplx<T> w3 = {(T)1, (T)0};
+  if (a.mode == 3) {
+    const int pc = (tid & 15) >> 2, kq = (int)ot * 4 + (tid & 3);
+    const cplx<T> t3 = a.tw[(a.O - 2) + ((pc * kq) & (int)(a.O - 1))];  // exp(-2 pi i k / n3)
+    w3 = {t3.re, -t3.im};
+  }
   for (int rr = rbeg; rr < rend; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
+    // mode 3: the pencil's axis-3 factor (k3 = oq is fixed along it) goes
+    // into the first pass with the axis-2 factors
+    const T g3 = a.mode == 3 ? ((f < a.fsplit ? a.gM : a.gV) + a.goff3 +
+                                (int64_t)(a.r0 + rr) * a.ksum)[act ? oq : 0]
+                             : (T)1;
     if (act) {
 #pragma unroll
       for (int u = 0; u < E / R1; ++u) {
         const int j = t + u * tpp;
         cplx<T> w[R1];
 #pragma unroll
-        for (int r = 0; r < R1; ++r) w[r] = cscale(x[u * R1 + r], gt[j + r * nb1]);
+        for (int r = 0; r < R1; ++r)
+          w[r] = cscale(x[u * R1 + r], a.mode == 3 ? gt[j + r * nb1] * g3 : gt[j + r * nb1]);
         dft_reg<R1, 1, T>(w);
 #pragma unroll
         for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
@@ -716,6 +745,30 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
         if (k < TKe)
           chunk_store(dst + (int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + o0 + j,
                       buf[((k << log2TJ) + j) * PL + pad8(e)]);
+      }
+    } else if (a.mode == 3) {
+      // radix-4 step of the axis-3 inverse (TJ == 16): entry j = 4 p + c of
+      // the run is class p of k' = 4 ot + c (the axis-3 factors are in)
+      const int j = tid & 15, pc = j >> 2, c = j & 3;
+      for (int id = tid; id < total; id += kThreads) {
+        const int e = (id >> 4) & (L - 1);
+        const int k = id >> (4 + LOG2L);
+        if (k < TKe) {
+          cplx<T> acc = {(T)0, (T)0};
+#pragma unroll
+          for (int sg = 0; sg < 4; ++sg) {
+            const cplx<T> y = buf[((k << log2TJ) + c + 4 * sg) * PL + pad8(e)];
+            switch ((pc * sg) & 3) {  // times i^(p s)
+              case 0: acc = {acc.re + y.re, acc.im + y.im}; break;
+              case 1: acc = {acc.re - y.im, acc.im + y.re}; break;
+              case 2: acc = {acc.re - y.re, acc.im - y.im}; break;
+              default: acc = {acc.re + y.im, acc.im - y.re}; break;
+            }
+          }
+          const cplx<T> out = {acc.re * w3.re - acc.im * w3.im, acc.re * w3.im + acc.im * w3.re};
+          chunk_store(dst + (int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + (int64_t)ot * 16 + j,
+                      out);
+        }
       }
     } else {
       for (int id = tid; id < total; id += kThreads) {
@@ -801,12 +854,15 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   if (a.O & (a.O - 1)) return false;
   a.TI = 1;
   a.mode = mode;
+  a.goff3 = g.N == 3 ? g.koff[2] : 0;
   const int per_block = (kThreads * Ef) / a.L;
   int hp = 1;
   while (hp < a.H1) hp <<= 1;
   // mode 2: TJ pencils along j3 give TJ-value contiguous runs of the output;
   // 16 (256-byte runs) measured 0.7 % faster than 8 at 64^3 and 128^3, 32 slower
-  a.TJ = mode == 2 ? (int)std::min<int64_t>(a.O, std::min(16, per_block)) : 1;
+  a.TJ = (mode == 2 || mode == 3) ? (int)std::min<int64_t>(a.O, std::min(16, per_block)) : 1;
+  // mode 3: 16 pencils (4 values of k' times the 4 quarters of axis 3)
+  if (mode == 3 && (a.TJ != 16 || g.N != 3 || a.O < 16 || (a.O & 15))) return false;
   a.TK = std::min(hp, per_block / a.TJ);
   if (mode == 1 && a.TK != g.TKy) return false;  // tile width fixed by the layout
   a.ktiles = (a.H1 + a.TK - 1) / a.TK;
@@ -1101,6 +1157,7 @@ struct VmrArgs {
   int log2TKy, rows3, log2n2, log2n3;
   int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
   int goff3;     // offset of grid axis 3 in the per-rung Gaussian tables
+  int xcd;       // k_slab3 SPLIT = 4: the 4 classes of a column on one XCD
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -1824,9 +1881,17 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // rung's fields are read once and not written again.  Eight waves keep every
 // lane of both transforms busy at 64 x 64 (64 pencils of 8 lanes, 8 rows of
 // 8 lanes per wave); one workgroup runs per CU.
+//
+// SPLIT = 4: the axis-2 pass stored the first radix-4 step of the axis-3
+// inverse (k_axis_ladder mode 3: its axis-3 factors applied, class p of
+// k' at [k1][j2][k' / 4][p][k' mod 4]); a workgroup per (rung, j2, p) takes
+// the n3 = (axis-3 length) / 4 values of class p of every pencil, runs the
+// rest of the axis-3 transform on them and writes the rows j3 = 4 m + p.  So
+// a grid whose (k1, k3) slab pair is 4x too large for LDS (128^3: 295 KB)
+// runs the same fused step on a quarter of it.
 constexpr int kSlabThreads = 512;
 
-template <int LOG2N1, int LOG2N3>
+template <int LOG2N1, int LOG2N3, int SPLIT>
 __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   constexpr int n1 = 1 << LOG2N1, H = n1 / 2, n3 = 1 << LOG2N3;
   constexpr int E = 8;
@@ -1841,6 +1906,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   constexpr int NS1 = n1 / LR1;
   static_assert(RW >= 1 && RW * lp1 <= 64 && NW * ppc3 >= 2 * H, "slab shape");
   static_assert(n1 > E && n3 > E, "slab axes longer than one radix-8 pass");
+  static_assert(SPLIT == 1 || (SPLIT == 4 && nb3 == 4), "the split layout has 4-value runs");
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<double>* P = (cplx<double>*)smem;       // [2][H][PL3]: the pair's planes
   cplx<double>* Zall = P + 2 * H * PL3;        // [NW][RW][PL1]: axis-1 rows
@@ -1850,7 +1916,25 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   for (int k = tid; k < n1; k += kSlabThreads) twl1[k] = a.tw[n1 - 2 + k];
   for (int k = tid; k < n3; k += kSlabThreads) twl3[k] = a.tw[n3 - 2 + k];
   const int n2 = a.n2r;
-  const int rr = (int)blockIdx.x / n2, j2 = (int)blockIdx.x - rr * n2;
+  int rr, j2, pcls = 0;
+  if constexpr (SPLIT == 1) {
+    rr = (int)blockIdx.x / n2;
+    j2 = (int)blockIdx.x - rr * n2;
+  } else {
+    // the 4 classes of one (rung, j2) on one XCD (workgroups go to the 8 XCDs
+    // in turn), so that their reads of the column's 256-byte runs share its L2
+    const uint32_t bid = blockIdx.x;
+    uint32_t colid;
+    if (a.xcd) {
+      pcls = (int)((bid >> 3) & 3u);
+      colid = ((bid >> 5) << 3) | (bid & 7u);
+    } else {
+      pcls = (int)(bid & 3u);
+      colid = bid >> 2;
+    }
+    rr = (int)colid / n2;
+    j2 = (int)colid - rr * n2;
+  }
   const int r = a.r0 + rr;
   const double* gm = a.gM + (int64_t)r * a.ksum;
   const double* gv = a.gV + (int64_t)r * a.ksum;
@@ -1861,19 +1945,23 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   const int pen = wv * ppc3 + lane / lp3, t3 = lane % lp3;
   const bool act3 = pen < 2 * H;
   const int pl = act3 ? pen / H : 0, k1p = act3 ? pen % H : 0;
-  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * n3 +
-                            (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q + t3;
+  // input i of this lane: k3 (or k' in class pcls) = t3 + i nb3
+  constexpr int istride = SPLIT == 1 ? nb3 : 16;
+  const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * (n3 * SPLIT) +
+                            (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q +
+                            (SPLIT == 1 ? t3 : pcls * 4 + t3);
   cplx<double>* pb3 = P + (act3 ? pen : 0) * PL3;
   double sf[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) sf[i] = (pl ? gv : gm)[a.goff3 + t3 + i * nb3];
+  for (int i = 0; i < E; ++i)
+    sf[i] = SPLIT == 1 ? (pl ? gv : gm)[a.goff3 + t3 + i * nb3] : 1.0;  // split: applied on store
   cplx<double> nx[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) nx[i] = {0.0, 0.0};
   auto load = [&](int bb) {
     if (act3) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) nx[i] = chunk_load(src + (int64_t)bb * a.fsize + i * nb3);
+      for (int i = 0; i < E; ++i) nx[i] = chunk_load(src + (int64_t)bb * a.fsize + i * istride);
     }
   };
   // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
@@ -1946,7 +2034,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   }
   if (!act1) return;
   const double sc = a.vscale[r];
-  const int64_t q = (int64_t)j2 + (int64_t)n2 * j3;  // grid row j2 + n2 j3
+  const int64_t q = (int64_t)j2 + (int64_t)n2 * (SPLIT * j3 + pcls);  // grid row j2 + n2 j3
   double* out = a.vmr + (int64_t)r * a.G + q * n1;
 #pragma unroll
   for (int u = 0; u < E; ++u) {
@@ -1979,24 +2067,34 @@ bool slab3_ok(const GridDev& g) {
          slab3_lds(g.n[0], g.n[2]) <= 160 * 1024;
 }
 
-template <int LOG2N1, int LOG2N3>
+// The split form: a 32-value quarter of axis 3 per workgroup (axis-3 length
+// 128), where the whole slab does not fit; the axis-2 pass (mode 3, 16
+// pencils per block) takes axis-2 lengths 16..128 in FP64.
+bool slab3_split_ok(const GridDev& g) {
+  return g.N == 3 && g.pack && g.pow2 && !slab3_ok(g) && g.n[2] == 128 && g.n[1] >= 16 &&
+         g.n[1] <= 128 && (g.log2n[0] >= 5 && g.log2n[0] <= 7) &&
+         slab3_shape(g.log2n[0], 5) && slab3_lds(g.n[0], 32) <= 160 * 1024;
+}
+
+template <int LOG2N1, int LOG2N3, int SPLIT>
 static void slab3_launch(unsigned blocks, hipStream_t s, const VmrArgs<double>& a) {
   constexpr int n1 = 1 << LOG2N1, n3 = 1 << LOG2N3;
   static const bool attr = [] {
     // more than 64 KB of LDS per workgroup; set once per instantiation
-    (void)hipFuncSetAttribute((const void*)k_slab3<LOG2N1, LOG2N3>,
+    (void)hipFuncSetAttribute((const void*)k_slab3<LOG2N1, LOG2N3, SPLIT>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)slab3_lds(n1, n3));
     return true;
   }();
   (void)attr;
-  k_slab3<LOG2N1, LOG2N3><<<blocks, kSlabThreads, slab3_lds(n1, n3), s>>>(a);
+  k_slab3<LOG2N1, LOG2N3, SPLIT><<<blocks, kSlabThreads, slab3_lds(n1, n3), s>>>(a);
 }
 
 bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride, int B,
                   int64_t n_samples, int R, int r0, const double* gM, const double* gV,
                   const double* cV, const double* vscale, const double* tw, const double* invk,
                   double* vmr, hipStream_t s) {
-  if (!slab3_ok(g)) return false;
+  const bool split = !slab3_ok(g);
+  if (split && !slab3_split_ok(g)) return false;
   VmrArgs<double> a{};
   a.src = src;
   a.src_rs = src_rung_stride;
@@ -2019,17 +2117,29 @@ bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_st
   a.Q = g.Q;
   a.n1 = g.n[0];
   a.n2r = g.n[1];
+  if (split) {
+    const int64_t cols = (int64_t)R * g.n[1];
+    a.xcd = cols % 8 == 0 ? 1 : 0;
+    const unsigned blocks = (unsigned)(cols * 4);
+    switch (g.log2n[0]) {
+      case 5: slab3_launch<5, 5, 4>(blocks, s, a); break;
+      case 6: slab3_launch<6, 5, 4>(blocks, s, a); break;
+      case 7: slab3_launch<7, 5, 4>(blocks, s, a); break;
+      default: return false;
+    }
+    return true;
+  }
   const unsigned blocks = (unsigned)((int64_t)R * g.n[1]);
   switch (g.log2n[0] * 16 + g.log2n[2]) {
-    case 4 * 16 + 6: slab3_launch<4, 6>(blocks, s, a); break;
-    case 5 * 16 + 5: slab3_launch<5, 5>(blocks, s, a); break;
-    case 5 * 16 + 6: slab3_launch<5, 6>(blocks, s, a); break;
-    case 6 * 16 + 4: slab3_launch<6, 4>(blocks, s, a); break;
-    case 6 * 16 + 5: slab3_launch<6, 5>(blocks, s, a); break;
-    case 6 * 16 + 6: slab3_launch<6, 6>(blocks, s, a); break;
-    case 7 * 16 + 4: slab3_launch<7, 4>(blocks, s, a); break;
-    case 7 * 16 + 5: slab3_launch<7, 5>(blocks, s, a); break;
-    case 8 * 16 + 4: slab3_launch<8, 4>(blocks, s, a); break;
+    case 4 * 16 + 6: slab3_launch<4, 6, 1>(blocks, s, a); break;
+    case 5 * 16 + 5: slab3_launch<5, 5, 1>(blocks, s, a); break;
+    case 5 * 16 + 6: slab3_launch<5, 6, 1>(blocks, s, a); break;
+    case 6 * 16 + 4: slab3_launch<6, 4, 1>(blocks, s, a); break;
+    case 6 * 16 + 5: slab3_launch<6, 5, 1>(blocks, s, a); break;
+    case 6 * 16 + 6: slab3_launch<6, 6, 1>(blocks, s, a); break;
+    case 7 * 16 + 4: slab3_launch<7, 4, 1>(blocks, s, a); break;
+    case 7 * 16 + 5: slab3_launch<7, 5, 1>(blocks, s, a); break;
+    case 8 * 16 + 4: slab3_launch<8, 4, 1>(blocks, s, a); break;
     default: return false;
   }
   return true;

@@ -146,6 +146,9 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // pair fits LDS: axis-3 and axis-1 inverse passes, VMR and Welford in one
 // kernel (FP64).  slab3_ok tells whether the grid takes it.
 bool slab3_ok(const GridDev& g);
+// Grids the split form takes (an axis-3 length of 128 whose whole slab does
+// not fit LDS): the axis-2 pass stores in mode 3, launch_slab3 runs quarters.
+bool slab3_split_ok(const GridDev& g);
 bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride, int B,
                   int64_t n_samples, int R, int r0, const double* gM, const double* gV,
                   const double* cV, const double* vscale, const double* tw, const double* invk,

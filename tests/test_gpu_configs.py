@@ -266,9 +266,10 @@ def test_c3_full_size_short_ladder_matches_oracle(pk):
 @pytest.mark.timeout(900)
 def test_c4_batch_matches_oracle(pk):
     """The 64-dataset mix in one batched submission (each request with its own
-    seed): the 1-D and 2-D items against the oracle on the same draws, the
-    64^3 items for their properties and against the same request estimated
-    alone through the plain path."""
+    seed): the 1-D and 2-D items among the first 24 datasets against the
+    oracle on the same draws (the rest for their properties, which keeps the
+    test within two minutes), the 64^3 items for their properties and against
+    the same request estimated alone through the plain path."""
     items = W.batch_mix_items()
     grids = {}
     req = []
@@ -291,6 +292,8 @@ def test_c4_batch_matches_oracle(pk):
                 alone = _gpu_estimate(pk, x, ranges, seed)
                 assert np.array_equal(alone.density, dens)
                 assert np.array_equal(alone.chosen_rung, chosen)
+            continue
+        if j >= 24:
             continue
         og = O.make_grid(*_spec(ranges))
         ref = O.gradepro(og, x, seed=seed, threads=THREADS)

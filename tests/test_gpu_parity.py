@@ -288,6 +288,14 @@ CASES = [
      60),
     ("3d-16x32x64", ([-4.0] * 3, [4.0] * 3, [16, 32, 64]), lambda: W.correlated_3d(20000, 39), 20,
      60),
+    # axis 3 of 128 points whose whole slab does not fit LDS: the axis-2 pass
+    # stores the first radix-4 step of axis 3, the slab kernel runs quarters
+    ("3d-64x16x128", ([-4.0] * 3, [4.0] * 3, [64, 16, 128]), lambda: W.correlated_3d(20000, 50), 20,
+     60),
+    ("3d-128x16x128", ([-4.0] * 3, [4.0] * 3, [128, 16, 128]), lambda: W.correlated_3d(20000, 51),
+     20, 60),
+    ("3d-32x32x128", ([-4.0] * 3, [4.0] * 3, [32, 32, 128]), lambda: W.correlated_3d(20000, 52), 20,
+     60),
     # more than three dimensions (the reference's CPU path takes any N)
     ("4d-16x16x8x8", ([-4.0] * 4, [4.0] * 4, [16, 16, 8, 8]), lambda: W.gaussian_nd(20000, 4, 28),
      20, 60),
@@ -405,6 +413,9 @@ VMR_CASES = [
      lambda: W.mixture_2d(8000, 25, lo=-4.5, hi=4.5)),
     # 64^3 (the fused slab kernel) over a box where the ladder passes two cells
     ("3d-64", ([-3.0] * 3, [3.0] * 3, [64, 64, 64]), lambda: W.correlated_3d(8000, 35, lo=-3.0, hi=3.0)),
+    # the slab kernel in quarters of a 128-point axis 3
+    ("3d-64x64x128", ([-2.5] * 3, [2.5] * 3, [64, 64, 128]),
+     lambda: W.correlated_3d(8000, 53, lo=-2.5, hi=2.5)),
 ]
 
 
