@@ -186,6 +186,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(int64_t n, int64_t np
   for (int k = threadIdx.x; k < nwords; k += kBinThreads) out[k] = words[k];
 }
 
+// LDS of k_bin_deposit: the tile's two accumulator blocks and the run
+// bounds of its tz + 1 planes.
+size_t deposit_lds(int T, int tz) {
+  return (size_t)2 * T * sizeof(unsigned long long) + (((size_t)2 * (tz + 1) * 4 + 7) & ~(size_t)7);
+}
+
 struct TileGeom {
   int ty, tz;            // tile rows along axes 2, 3 (axis 1 spans the grid)
   int tiles_y, tiles_z;  // tiles per axis
@@ -200,6 +206,14 @@ struct TileGeom {
 // indices) before depositing any of them, so that one trip waits for one
 // round of memory latency rather than kBatch: with one 128 KB workgroup per
 // CU the loop is latency-bound.
+//
+// Blocks run tile-major (block = tile (B + 1) + b).  Workgroups are handed to
+// the XCDs, and within an XCD to its shader engines, in turn, and each
+// engine takes its next workgroup in order; with bootstrap-major blocks every
+// bootstrap of one tile went to the same engine, so the engine of a dense
+// tile (the grid's centre: 15x the samples of an edge tile) ran all of that
+// tile's workgroups in series while the rest of its XCD waited behind it in
+// the dispatch order.  Tile-major blocks spread each tile over every engine.
 constexpr int kBatch = 4;
 
 template <int N>
@@ -216,8 +230,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, T
                                                              unsigned long long* __restrict__ acc) {
   extern __shared__ unsigned long long lacc[];
   const int ntiles = tg.tiles_y * tg.tiles_z;
-  const int b = blockIdx.x / ntiles;
-  const int tile = blockIdx.x - b * ntiles;
+  const int tile = blockIdx.x / (B + 1);
+  const int b = blockIdx.x - tile * (B + 1);
   const int tyi = tile % tg.tiles_y, tzi = tile / tg.tiles_y;
   const int n1 = g.n[0], n2 = N >= 2 ? g.n[1] : 1;
   const int y0 = tyi * tg.ty, z0 = tzi * tg.tz;
@@ -225,21 +239,27 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, T
   const bool sq = b < B;
   unsigned long long* lM = lacc;
   unsigned long long* lV = lacc + T;
+  // [plane][2]: the sorted run of each plane's rows [ylo, yhi)
+  int32_t* lrun = (int32_t*)(lacc + 2 * T);
+  const int n3 = N == 3 ? g.n[2] : 1;
+  const int zlo = N == 3 ? std::max(0, z0 - 1) : 0;
+  const int zhi = N == 3 ? std::min(z0 + tg.tz, n3) : 1;  // exclusive
+  const int ylo = N >= 2 ? std::max(0, y0 - 1) : 0;
+  const int yhi = N >= 2 ? std::min(y0 + tg.ty, n2) : 1;
   for (int k = threadIdx.x; k < 2 * T; k += kBinThreads) lacc[k] = 0ull;
+  // every plane's run bounds in one round of loads, not one round per plane
+  for (int k = threadIdx.x; k < 2 * (zhi - zlo); k += kBinThreads) {
+    const int64_t c = ((int64_t)(zlo + (k >> 1)) * n2 + ((k & 1) ? yhi : ylo)) * n1;
+    lrun[k] = start[std::min<int64_t>(c, g.G)];
+  }
+  (void)ntiles;
   __syncthreads();
   const bool skip = sq && ovf[b] != 0;  // binned by the per-draw kernel instead
   if (!skip) {
-    const int n3 = N == 3 ? g.n[2] : 1;
-    const int zlo = N == 3 ? std::max(0, z0 - 1) : 0;
-    const int zhi = N == 3 ? std::min(z0 + tg.tz, n3) : 1;  // exclusive
-    const int ylo = N >= 2 ? std::max(0, y0 - 1) : 0;
-    const int yhi = N >= 2 ? std::min(y0 + tg.ty, n2) : 1;
     const uint8_t* cntb = cnt + (int64_t)b * npad;
     for (int z = zlo; z < zhi; ++z) {
       // base cells of rows [ylo, yhi) of plane z: one contiguous run
-      const int64_t cA = ((int64_t)z * n2 + ylo) * n1;
-      const int64_t cB = ((int64_t)z * n2 + yhi) * n1;
-      const int32_t pA = start[cA], pB = start[std::min<int64_t>(cB, g.G)];
+      const int32_t pA = lrun[2 * (z - zlo)], pB = lrun[2 * (z - zlo) + 1];
       for (int32_t p0 = pA + threadIdx.x; p0 < pB; p0 += kBatch * kBinThreads) {
         int32_t cl[kBatch], ord[kBatch];
         double rh[kBatch][N];
@@ -309,7 +329,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_deposit(GridDev g, int B, T
   __syncthreads();
   // write the tile: rows y0.., planes z0.. (edge tiles may be partial)
   const int64_t fM = sq ? b : 2 * (int64_t)B;
-  const int n3 = N == 3 ? g.n[2] : 1;
   for (int k = threadIdx.x; k < T; k += kBinThreads) {
     const int x = k % n1;
     const int rest = k / n1;
@@ -458,6 +477,8 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
   if (g.N > 3 || binsort_bytes(g, n, B) > work_bytes) return -1;
   if (tile_cells <= 0 || tile_cells > kTileCells) tile_cells = kTileCells;
   if (g.G + 2 >= (1ll << 31) || n >= (1ll << 31) || g.n[0] > tile_cells) return -1;
+  const TileGeom tg = tile_geom(g, tile_cells);
+  if (deposit_lds(tg.T, tg.tz) > 160 * 1024) return -1;
   const int64_t npad = (n + 3) & ~(int64_t)3;
   unsigned char* p = (unsigned char*)work;
   auto take = [&](uint64_t v) {
@@ -491,7 +512,7 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
                           (const void*)k_bin_deposit<3>};
     for (const void* f : dep)
       (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                2 * kTileCells * (int)sizeof(unsigned long long));
+                                160 * 1024);
     return true;
   }();
   (void)attrs;
@@ -512,9 +533,8 @@ int launch_binning_sorted(const GridDev& g, const double* x, int64_t n, const in
                                                                    key1, cnt, ovf, err);
   }
   // 3. tile deposits for every bootstrap and the full sample
-  const TileGeom tg = tile_geom(g, tile_cells);
   const unsigned nblk = (unsigned)((B + 1) * tg.tiles_y * tg.tiles_z);
-  const size_t lds = (size_t)2 * tg.T * sizeof(unsigned long long);
+  const size_t lds = deposit_lds(tg.T, tg.tz);
   auto deposit = g.N == 1 ? k_bin_deposit<1> : g.N == 2 ? k_bin_deposit<2> : k_bin_deposit<3>;
   deposit<<<nblk, kBinThreads, lds, s>>>(g, B, tg, start, scell, order, srh, cnt, npad, ovf, denom,
                                          scaleM, scaleV, (unsigned long long*)acc);

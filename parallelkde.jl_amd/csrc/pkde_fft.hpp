@@ -524,6 +524,130 @@ __device__ __forceinline__ void wave_fft(cplx<T>* pb, int t, bool active,
   wave_fft_from<LOG2L, 0, E, DIR, T>(pb, t, active, tw);
 }
 
+// ------------------------------- wave-local passes, twiddles in registers
+// The same passes as wave_fft_from / wave_fft_to_regs for a lane that keeps
+// the twiddles of every pass after the first in registers (loaded once with
+// wave_tw_load, outside a loop over many transforms of the same length), so
+// that the passes read only data from LDS.  twc holds, pass by pass, for
+// each butterfly u of the lane and r = 1..R-1, tw[jm r L/(Ns R)] (conjugated
+// for the forward direction), exactly the factors the LDS-table passes use.
+template <int LOG2L, int LOG2NS, int E>
+__host__ __device__ constexpr int wave_tw_count() {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  if constexpr (rem <= 0) {
+    return 0;
+  } else {
+    constexpr int LR = rem >= LE ? LE : rem;
+    constexpr int R = 1 << LR;
+    return (E / R) * (R - 1) + wave_tw_count<LOG2L, LOG2NS + LR, E>();
+  }
+}
+
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T, int OFF, int NT>
+__device__ __forceinline__ void wave_tw_load(int t, const cplx<T>* __restrict__ tw,
+                                             cplx<T> (&twc)[NT]) {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  if constexpr (rem > 0) {
+    constexpr int LR = rem >= LE ? LE : rem;
+    constexpr int R = 1 << LR;
+    constexpr int L = 1 << LOG2L, tpp = L / E, Ns = 1 << LOG2NS;
+    constexpr int twstep = L / (Ns * R);
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int jm = (t + u * tpp) & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> f = tw[jm * r * twstep];
+        if (DIR > 0) f.im = -f.im;
+        twc[OFF + u * (R - 1) + r - 1] = f;
+      }
+    }
+    wave_tw_load<LOG2L, LOG2NS + LR, E, DIR, T, OFF + (E / R) * (R - 1), NT>(t, tw, twc);
+  }
+}
+
+template <int R, int E, int LOG2L, int LOG2NS, int DIR, typename T, int OFF, int NT>
+__device__ __forceinline__ void wave_pass_rt(cplx<T>* pb, int t, bool active,
+                                             const cplx<T> (&twc)[NT]) {
+  constexpr int L = 1 << LOG2L;
+  constexpr int tpp = L / E;
+  constexpr int nbfly = L / R;
+  constexpr int Ns = 1 << LOG2NS;
+  constexpr int LOG2R = ilog2c(R);
+  static_assert(Ns > 1, "the first pass has no twiddles");
+  cplx<T> v[E];
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = t + u * tpp;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u * R + r] = pb[pad8(j + r * nbfly)];
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = t + u * tpp;
+      const int jm = j & (Ns - 1);
+      cplx<T>* w = v + u * R;
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twc[OFF + u * (R - 1) + r - 1]);
+      dft_reg<R, DIR, T>(w);
+      const int out0 = ((j >> LOG2NS) << (LOG2NS + LOG2R)) + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pb[pad8(out0 + r * Ns)] = w[r];
+    }
+  }
+  wave_sync();
+}
+
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T, int OFF, int NT>
+__device__ __forceinline__ void wave_fft_from_rt(cplx<T>* pb, int t, bool active,
+                                                 const cplx<T> (&twc)[NT]) {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  if constexpr (rem > 0) {
+    constexpr int LR = rem >= LE ? LE : rem;
+    constexpr int R = 1 << LR;
+    wave_pass_rt<R, E, LOG2L, LOG2NS, DIR, T, OFF, NT>(pb, t, active, twc);
+    wave_fft_from_rt<LOG2L, LOG2NS + LR, E, DIR, T, OFF + (E / R) * (R - 1), NT>(pb, t, active,
+                                                                                 twc);
+  }
+}
+
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T, int OFF, int NT>
+__device__ __forceinline__ void wave_fft_to_regs_rt(cplx<T>* pb, int t, bool active,
+                                                    const cplx<T> (&twc)[NT], cplx<T>* out) {
+  constexpr int LE = ilog2c(E);
+  constexpr int rem = LOG2L - LOG2NS;
+  static_assert(rem > 0 && LOG2NS > 0, "a pass with twiddles left");
+  if constexpr (rem > LE) {
+    wave_pass_rt<E, E, LOG2L, LOG2NS, DIR, T, OFF, NT>(pb, t, active, twc);
+    wave_fft_to_regs_rt<LOG2L, LOG2NS + LE, E, DIR, T, OFF + (E - 1), NT>(pb, t, active, twc,
+                                                                          out);
+  } else {
+    constexpr int R = 1 << rem;
+    constexpr int L = 1 << LOG2L;
+    constexpr int tpp = L / E;
+    constexpr int nbfly = L / R;
+    if (active) {
+#pragma unroll
+      for (int u = 0; u < E / R; ++u) {
+        const int j = t + u * tpp;
+        cplx<T>* w = out + u * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) w[r] = pb[pad8(j + r * nbfly)];
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twc[OFF + u * (R - 1) + r - 1]);
+        dft_reg<R, DIR, T>(w);
+      }
+    }
+  }
+}
+
 // ------------------------------------------- wave-local mixed-radix passes
 // Radix-3 and radix-5 butterflies with the roots taken from the length-L
 // table (w_m = tw[m L / R], conjugated for the inverse), in the symmetric

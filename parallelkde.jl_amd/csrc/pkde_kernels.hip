@@ -1890,6 +1890,11 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // a grid whose (k1, k3) slab pair is 4x too large for LDS (128^3: 295 KB)
 // runs the same fused step on a quarter of it.
 constexpr int kSlabThreads = 512;
+// Twiddles of the axis-3 / axis-1 passes held in registers when a lane needs
+// at most this many (else read from the LDS table in every pass); axis 1
+// only for 64-point rows, where they fit the registers without spilling.
+constexpr int kSlabRegTw3 = 8;
+constexpr int kSlabRegTw1 = 8;
 
 template <int LOG2N1, int LOG2N3, int SPLIT>
 __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
@@ -1992,6 +1997,14 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     mean[u] = 0.0;
     m2[u] = 0.0;
   }
+  // loop-invariant twiddles of the passes after the first, in registers
+  constexpr int NT3 = wave_tw_count<LOG2N3, 3, E>(), NT1 = wave_tw_count<LOG2N1, 3, E>();
+  constexpr bool rt3 = NT3 <= kSlabRegTw3, rt1 = NT1 <= kSlabRegTw1 && LOG2N1 == 6;
+  cplx<double> tw3r[rt3 ? NT3 : 1], tw1r[rt1 ? NT1 : 1];
+  if constexpr (rt3)
+    wave_tw_load<LOG2N3, 3, E, 1, double, 0, NT3>(t3, (const cplx<double>*)a.tw + n3 - 2, tw3r);
+  if constexpr (rt1)
+    wave_tw_load<LOG2N1, 3, E, 1, double, 0, NT1>(t1, (const cplx<double>*)a.tw + n1 - 2, tw1r);
   load(0);
   for (int b = 0; b < B; ++b) {
     const double inv = a.invk[b];
@@ -2001,7 +2014,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     load(b + 1 < B ? b + 1 : b);  // in flight while this bootstrap is transformed
     __syncthreads();  // every wave is done reading the previous planes
     wave_first_pass<LOG2N3, E, 1, double>(pb3, t3, act3, v);
-    wave_fft_from<LOG2N3, 3, E, 1, double>(pb3, t3, act3, twl3);
+    if constexpr (rt3)
+      wave_fft_from_rt<LOG2N3, 3, E, 1, double, 0, NT3>(pb3, t3, act3, tw3r);
+    else
+      wave_fft_from<LOG2N3, 3, E, 1, double>(pb3, t3, act3, twl3);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -2020,7 +2036,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
       v[E / 2] = {pa.im * fm[E / 2], pq.im * fv[E / 2]};
     }
     wave_first_pass<LOG2N1, E, 1, double>(pb1, t1, act1, v);
-    wave_fft_to_regs<LOG2N1, 3, E, 1, double>(pb1, t1, act1, twl1, v);
+    if constexpr (rt1)
+      wave_fft_to_regs_rt<LOG2N1, 3, E, 1, double, 0, NT1>(pb1, t1, act1, tw1r, v);
+    else
+      wave_fft_to_regs<LOG2N1, 3, E, 1, double>(pb1, t1, act1, twl1, v);
     if (act1) {
 #pragma unroll
       for (int u = 0; u < E; ++u) {
