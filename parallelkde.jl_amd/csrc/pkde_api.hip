@@ -146,7 +146,11 @@ static int y_units(const pkde_plan* pl) { return pl->g.N == 3 && !use_slab3(pl) 
 // its last, partly filled round) and reads the spectra once: at 256^2 the
 // ladder takes 11.97 ms with 4 chunks against 12.38 ms with 7; at 64^3
 // 45.9 ms with 7 chunks (16 GiB) against 48.4 ms with 14 (8 GiB), and no
-// less with 32 GiB.
+// less with 32 GiB.  A plan of its own (not one lent a stream's chunk buffer)
+// whose rungs are large takes at least 16 rungs per chunk when they fit in a
+// quarter of the device memory: at 128^3 (3.4 GB per rung) 5-rung chunks
+// read the spectra 51 times per ladder, and the ladder took 514 ms against
+// 485 ms with 19-rung chunks.
 static uint64_t y_part_bytes(const pkde_plan* pl) {
   const size_t rs = real_size(pl->prec);
   const uint64_t per_rung = (uint64_t)2 * pl->B * pl->g.fsY * 2 * rs;
@@ -159,6 +163,8 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   // at least 4 rungs when they fit in 1/8 of the device memory
   if (!pl->ctx->chunk_fixed)
     rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(4, pl->ctx->mem_total / 8 / units / per_rung));
+  if (!pl->ctx->chunk_fixed && !pl->chunk_budget)
+    rungs = std::max<uint64_t>(rungs, std::min<uint64_t>(16, pl->ctx->mem_total / 4 / units / per_rung));
   const uint64_t S = (uint64_t)pl->S;
   rungs = std::min<uint64_t>(rungs, S);
   const uint64_t chunks = (S + rungs - 1) / rungs;
