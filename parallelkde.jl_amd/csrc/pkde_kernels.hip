@@ -1895,6 +1895,14 @@ constexpr int kSlabThreads = 512;
 // only for 64-point rows, where they fit the registers without spilling.
 constexpr int kSlabRegTw3 = 8;
 constexpr int kSlabRegTw1 = 8;
+// Bootstraps whose planes are in flight while one is transformed: two for
+// the 64 x 64 slab (64^3 ladder 46.3 -> 43.4 ms, 254 VGPRs, no spills), one
+// elsewhere (the other shapes spill at two; 128^3 in quarters 490 -> 505 ms,
+// 32^3 slower too).
+template <int LOG2N1, int LOG2N3, int SPLIT>
+constexpr int slab_depth() {
+  return SPLIT == 1 && LOG2N1 == 6 && LOG2N3 == 6 ? 2 : 1;
+}
 
 template <int LOG2N1, int LOG2N3, int SPLIT>
 __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
@@ -1960,13 +1968,13 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
 #pragma unroll
   for (int i = 0; i < E; ++i)
     sf[i] = SPLIT == 1 ? (pl ? gv : gm)[a.goff3 + t3 + i * nb3] : 1.0;  // split: applied on store
-  cplx<double> nx[E];
+  cplx<double> nx[E], ny[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) nx[i] = {0.0, 0.0};
-  auto load = [&](int bb) {
+  for (int i = 0; i < E; ++i) nx[i] = ny[i] = {0.0, 0.0};
+  auto load = [&](cplx<double>(&dst)[E], int bb) {
     if (act3) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) nx[i] = chunk_load(src + (int64_t)bb * a.fsize + i * istride);
+      for (int i = 0; i < E; ++i) dst[i] = chunk_load(src + (int64_t)bb * a.fsize + i * istride);
     }
   };
   // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
@@ -2005,13 +2013,16 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     wave_tw_load<LOG2N3, 3, E, 1, double, 0, NT3>(t3, (const cplx<double>*)a.tw + n3 - 2, tw3r);
   if constexpr (rt1)
     wave_tw_load<LOG2N1, 3, E, 1, double, 0, NT1>(t1, (const cplx<double>*)a.tw + n1 - 2, tw1r);
-  load(0);
-  for (int b = 0; b < B; ++b) {
+  // one bootstrap: its planes were loaded into cur kDepth bootstraps ahead;
+  // cur then takes the loads of bootstrap b + kDepth (or b's again at the end:
+  // valid addresses, values unused)
+  constexpr int kSlabDepth = slab_depth<LOG2N1, LOG2N3, SPLIT>();
+  auto step = [&](int b, cplx<double>(&cur)[E]) {
     const double inv = a.invk[b];
     cplx<double> v[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = cscale(nx[i], sf[i]);
-    load(b + 1 < B ? b + 1 : b);  // in flight while this bootstrap is transformed
+    for (int i = 0; i < E; ++i) v[i] = cscale(cur[i], sf[i]);
+    load(cur, b + kSlabDepth < B ? b + kSlabDepth : b);
     __syncthreads();  // every wave is done reading the previous planes
     wave_first_pass<LOG2N3, E, 1, double>(pb3, t3, act3, v);
     if constexpr (rt3)
@@ -2049,6 +2060,17 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
         mean[u] += d * inv;
         m2[u] += d * (xv - mean[u]);
       }
+    }
+  };
+  if constexpr (kSlabDepth == 1) {
+    load(nx, 0);
+    for (int b = 0; b < B; ++b) step(b, nx);
+  } else {
+    load(nx, 0);
+    load(ny, B > 1 ? 1 : 0);
+    for (int b = 0; b < B; b += 2) {
+      step(b, nx);
+      if (b + 1 < B) step(b + 1, ny);
     }
   }
   if (!act1) return;
