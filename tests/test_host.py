@@ -219,3 +219,36 @@ def test_integration_notes_state_the_header_abi():
     notes = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     stated = {int(v) for v in re.findall(r"ABI version[^0-9]{0,40}?(\d+)", notes)}
     assert stated == {abi}, stated
+
+
+def _build_c_caller(tmp_path):
+    """Compiles tests/c_abi_caller.c with the host C compiler against the
+    public header and the built library (what a C host does)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    libdir = os.path.join(ROOT, "parallelkde.jl_amd", "lib")
+    exe = tmp_path / "c_abi_caller"
+    subprocess.run([cc, "-O2", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-o", str(exe), os.path.join(ROOT, "tests", "c_abi_caller.c"),
+                    "-L", libdir, "-lparallelkde_hip", f"-Wl,-rpath,{libdir}", "-lm"], check=True)
+    return exe
+
+
+def test_c_caller_builds_and_reports_no_device(tmp_path):
+    """A plain C program compiles against include/parallelkde_hip.h and links
+    the library; without a GPU it gets PKDE_NO_DEVICE from
+    pkde_context_create and exits 3 (the glue's cue for the :cpu fallback)."""
+    import subprocess
+    from parallelkde_rocm import _lib
+    h = C.c_void_p()
+    rc = _lib.load().pkde_context_create(0, C.byref(h))
+    if rc == 0:
+        _lib.load().pkde_context_destroy(h)
+        pytest.skip("a GPU is present (the GPU test runs the program)")
+    exe = _build_c_caller(tmp_path)
+    r = subprocess.run([str(exe), str(tmp_path / "out.bin")], capture_output=True, text=True)
+    assert r.returncode == 3, (r.returncode, r.stdout, r.stderr)
+    assert "no usable GPU" in r.stdout
