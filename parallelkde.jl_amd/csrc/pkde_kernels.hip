@@ -1391,6 +1391,132 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
   }
 }
 
+// 1-D ladder for 1024-point grids in FP64, one block of kWavesMs wavefronts
+// per rung.  A 1024-point pencil pair per wave does not fit LDS (2 pencils
+// of 18 KB per wave), so each wavefront takes ONE bootstrap b per round and
+// transforms z = m + i s, the sums field m and the squares field s of that
+// bootstrap (both real), as one complex pencil: the real part of the result
+// is the sums field and the imaginary part the squares field, the two terms
+// of b's VMR ratio.  The transform has 16 values per lane (radix-16
+// passes); 7 wavefronts leave room in LDS for the twiddle table.  After a
+// barrier every thread runs the Welford updates of its points over the
+// round's bootstraps in order.  Same arithmetic as k_ladder_1d otherwise.
+constexpr int kWavesMs = 7;
+
+template <int LOG2N1>
+__global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> a) {
+  constexpr int n1 = 1 << LOG2N1;
+  constexpr int H = n1 / 2;
+  constexpr int E = n1 / 64;  // one pencil per wavefront
+  constexpr int PL = pencil_stride(n1);
+  constexpr int NT = 64 * kWavesMs;
+  constexpr int PPT = (n1 + NT - 1) / NT;  // points per thread in the Welford phase
+  static_assert(E == 16, "1024-point pencils, 16 values per lane");
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cplx<double>* twl = (cplx<double>*)smem + kWavesMs * PL;
+  for (int k = tid; k < n1; k += NT) twl[k] = a.tw[n1 - 2 + k];
+  __syncthreads();
+  cplx<double>* Z = (cplx<double>*)smem + wv * PL;
+  const int r = a.r0 + (int)blockIdx.x;
+  const double* gm1 = a.gM + (int64_t)r * a.ksum;
+  const double* gv1 = a.gV + (int64_t)r * a.ksum;
+  const cplx<double>* src = a.src + (int64_t)blockIdx.x * a.src_rs;
+  const double vfac = a.cV[r] * a.invG;
+  const int B = a.B;
+  double mean[PPT], m2[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    mean[u] = 0.0;
+    m2[u] = 0.0;
+  }
+  // The wave's loads of a round (the half spectra of its bootstrap's sums
+  // and squares fields) go to registers all at once, one round ahead.  A
+  // wave without a bootstrap in the round reloads bootstrap 0 (valid
+  // addresses, values unused).
+  constexpr int NIT = (2 * (H + 1) + 63) / 64;
+  cplx<double> la[NIT];
+  auto issue = [&](int b) {
+    const int bb = b < B ? b : 0;
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int id = lane + 64 * i;
+      const int idc = id < 2 * (H + 1) ? id : 0;
+      const int ty = idc > H;
+      const int k1 = ty ? idc - H - 1 : idc;
+      la[i] = src[(int64_t)(ty * B + bb) * a.fsize + k1];
+    }
+  };
+  issue(wv);
+  for (int b0 = 0; b0 < B; b0 += kWavesMs) {
+    const int b = b0 + wv;
+    if (b < B) {
+      // Z[k] = M[k] + i S[k], Z[n1 - k] = conj(M[k]) + i conj(S[k]), with M
+      // the sums half spectrum and S the squares one, each with its factor:
+      // the sums first, then the squares added in
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int id = lane + 64 * i;
+        if (id <= H) {
+          const cplx<double> M = cscale(la[i], gm1[id]);
+          Z[pad8(id)] = M;
+          if (id > 0 && id < H) Z[pad8(n1 - id)] = cconj(M);
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int id = lane + 64 * i;
+        if (id > H && id < 2 * (H + 1)) {
+          const int k1 = id - H - 1;
+          const cplx<double> S = cscale(la[i], gv1[k1]);
+          const cplx<double> m = Z[pad8(k1)];
+          Z[pad8(k1)] = {m.re - S.im, m.im + S.re};
+          if (k1 > 0 && k1 < H) {
+            const cplx<double> mc = Z[pad8(n1 - k1)];
+            Z[pad8(n1 - k1)] = {mc.re + S.im, mc.im + S.re};
+          }
+        }
+      }
+      wave_sync();
+      wave_fft<LOG2N1, 1, double, E>(Z, lane, true, twl);
+      // the ratio of bootstrap b replaces the pencil (as doubles in the
+      // first 8 bytes of each slot; every lane rewrites what it read)
+      for (int j = lane; j < n1; j += 64) {
+        const cplx<double> z = Z[pad8(j)];
+        *(double*)(Z + pad8(j)) = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
+      }
+    }
+    issue(b + kWavesMs);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int j = tid + u * NT;
+      if (j < n1) {
+        for (int w = 0; w < kWavesMs; ++w) {
+          const int bw = b0 + w;
+          if (bw >= B) break;
+          const cplx<double>* Zw = (const cplx<double>*)smem + w * PL;
+          const double xv = *(const double*)(Zw + pad8(j));
+          const double d = xv - mean[u];
+          mean[u] += d / (double)(bw + 1);
+          m2[u] += d * (xv - mean[u]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double sc = a.vscale[r];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int j = tid + u * NT;
+    if (j < n1) {
+      const double vv = sc * m2[u] / (double)(B - 1);
+      a.vmr[(int64_t)r * a.G + j] = isfinite(vv) ? log(vv) : (double)NAN;
+    }
+  }
+}
+
 // The same for lengths that are not powers of two but fit the wave-local
 // mixed-radix FFT (a.fac: radices 2, 3, 4, 5, 8; n1 <= 512): PPT = points per
 // thread of the Welford phase.
@@ -1542,9 +1668,19 @@ static bool ladder_1d_dispatch(int log2n1, unsigned blocks, hipStream_t s, const
     case 10:
       if constexpr (sizeof(T) == 4) {
         ladder_1d_launch<T, 10>(blocks, s, a);
-        return true;
+      } else {
+        constexpr size_t lds =
+            ((size_t)kWavesMs * pencil_stride(1024) + 1024) * sizeof(cplx<double>);
+        static const bool attr = [] {
+          // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
+          (void)hipFuncSetAttribute((const void*)k_ladder_1d_ms<10>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          return true;
+        }();
+        (void)attr;
+        k_ladder_1d_ms<10><<<blocks, 64 * kWavesMs, lds, s>>>(a);
       }
-      return false;
+      return true;
     default: return false;
   }
 }

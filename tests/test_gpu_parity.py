@@ -260,6 +260,9 @@ def _check_fp64(m, envelope):
 CASES = [
     ("1d-512", ([-5.0], [5.0], [512]), lambda: W.gaussian_1d(10000, 1), 100, 250),
     ("2d-64", ([-6.0, -6.0], [6.0, 6.0], [64, 64]), lambda: W.mixture_2d(20000, 2), 40, 120),
+    # 1024 points: one bootstrap's sums and squares fields in one pencil
+    ("1d-1024", ([-6.0], [6.0], [1024]), lambda: W.gaussian_1d(20000, 41, lo=-6.0, hi=6.0), 40,
+     120),
     ("3d-32", ([-4.0] * 3, [4.0] * 3, [32, 32, 32]), lambda: W.correlated_3d(20000, 3), 20, 60),
     # lengths that are not powers of two (mixed-radix kernels): the
     # reference's own 300-point test grid, odd and mixed axes
@@ -405,6 +408,7 @@ def test_estimate_options_match_oracle(pk, case):
 
 VMR_CASES = [
     ("1d-256", ([-5.0], [5.0], [256]), lambda: W.gaussian_1d(5000, 4)),
+    ("1d-1024", ([-3.0], [3.0], [1024]), lambda: W.gaussian_1d(20000, 42, lo=-3.0, hi=3.0)),
     ("1d-300", ([-5.0], [5.0], [300]), lambda: W.gaussian_1d(5000, 4)),
     ("2d-150x96", ([-6.0, -6.0], [6.0, 6.0], [150, 96]), lambda: W.mixture_2d(8000, 7)),
     # 101 points (prime) over a span where the ladder passes two cells
