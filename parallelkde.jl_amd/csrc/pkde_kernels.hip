@@ -642,10 +642,19 @@ __global__ __launch_bounds__(kThreads) void k_axis_ladder(AxisArgs<T> a) {
   const int sp = (int)(bid % (uint32_t)a.nsplit);
   bid /= (uint32_t)a.nsplit;
   const int rbeg = sp * a.R / a.nsplit, rend = (sp + 1) * a.R / a.nsplit;
-  const uint32_t kt = bid % (uint32_t)a.ktiles;
-  bid /= (uint32_t)a.ktiles;
-  const uint32_t ot = bid & ((1u << a.log2otiles) - 1u);
-  bid >>= a.log2otiles;
+  uint32_t kt, ot;
+  if (a.mode == 3) {
+    // outer tiles fastest: blocks that run together fill one k1 column
+    ot = bid & ((1u << a.log2otiles) - 1u);
+    bid >>= a.log2otiles;
+    kt = bid % (uint32_t)a.ktiles;
+    bid /= (uint32_t)a.ktiles;
+  } else {
+    kt = bid % (uint32_t)a.ktiles;
+    bid /= (uint32_t)a.ktiles;
+    ot = bid & ((1u << a.log2otiles) - 1u);
+    bid >>= a.log2otiles;
+  }
   const int f = a.fid0 + (int)bid;
   const int log2TK = a.log2TK, log2TJ = a.log2TJ;
   const int TK = 1 << log2TK, TJ = 1 << log2TJ;
