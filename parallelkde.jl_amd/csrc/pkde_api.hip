@@ -157,8 +157,11 @@ static uint64_t y_part_bytes(const pkde_plan* pl) {
   const uint64_t units = (uint64_t)y_units(pl);
   if (per_rung == 0) return (uint64_t)fsize(pl) * 2 * rs * 2;  // :rot: the means pass only
   uint64_t budget = pl->chunk_budget ? pl->chunk_budget : pl->ctx->chunk_bytes;
+  // the context's default budget is capped at 1/16 of device memory; a
+  // budget given to the plan (a stream lane's or a server slot's, sized by
+  // the caller for its streams) at 1/4
   if (!pl->ctx->chunk_fixed && pl->ctx->mem_total)
-    budget = std::min<uint64_t>(budget, pl->ctx->mem_total / 16);
+    budget = std::min<uint64_t>(budget, pl->ctx->mem_total / (pl->chunk_budget ? 4 : 16));
   uint64_t rungs = std::max<uint64_t>(1, budget / units / per_rung);
   // at least 4 rungs when they fit in 1/8 of the device memory
   if (!pl->ctx->chunk_fixed)

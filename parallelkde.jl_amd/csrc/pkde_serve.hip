@@ -93,20 +93,26 @@ struct Footprint {
   int status = PKDE_OK;
   uint64_t plan = 0;   // plan buffers, chunk buffer excluded
   uint64_t chunk = 0;  // the lent chunk buffer
+  uint64_t chunk_own = 0;  // the chunk buffer a plan of its own would take
   int64_t G = 0;
   double work = 0.0;   // estimated device work (ladder + binning)
+  double ladder_work = 0.0;  // G S B: grid-point rung bootstraps of the ladder
 };
 
-Footprint footprint_of(pkde_ctx* ctx, const pkde_request& r, int64_t n_cap) {
+Footprint footprint_of(pkde_ctx* ctx, const pkde_request& r, int64_t n_cap,
+                       uint64_t chunk_budget) {
   Footprint f;
   pkde_plan hp;
   f.status = plan_setup_host(ctx, r.ndims, r.n_per_axis, r.lo, r.hi, r.h, &r.params, &hp);
   if (f.status) return f;
-  hp.chunk_budget = ctx->lane_chunk_bytes;
+  hp.chunk_budget = 0;
+  f.chunk_own = plan_chunk_bytes(&hp);
+  hp.chunk_budget = chunk_budget;
   f.plan = plan_bytes(&hp, n_cap, false);
   f.chunk = plan_chunk_bytes(&hp);
   f.G = hp.g.G;
-  f.work = (double)hp.g.G * hp.S * hp.B + (double)r.n * hp.B * (double)(1 << r.ndims);
+  f.ladder_work = (double)hp.g.G * hp.S * hp.B;
+  f.work = f.ladder_work + (double)r.n * hp.B * (double)(1 << r.ndims);
   return f;
 }
 
@@ -118,10 +124,11 @@ int check_request(const pkde_request& r) {
 
 // A plan for a stream lane or slot: the lane chunk budget, the lane's chunk
 // buffer lent to it.
-int make_lane_plan(pkde_ctx* ctx, const pkde_request& r, pkde_plan** out) {
+int make_lane_plan(pkde_ctx* ctx, const pkde_request& r, pkde_plan** out,
+                   uint64_t chunk_budget) {
   const int rc =
       pkde_gradepro_plan_create(ctx, r.ndims, r.n_per_axis, r.lo, r.hi, r.h, &r.params, out);
-  if (rc == PKDE_OK) (*out)->chunk_budget = ctx->lane_chunk_bytes;
+  if (rc == PKDE_OK) (*out)->chunk_budget = chunk_budget;
   return rc;
 }
 
@@ -167,7 +174,7 @@ static int estimate_batch_impl(pkde_ctx* ctx, int32_t count, const pkde_request*
   for (int i = 0; i < count; ++i) {
     st[i] = check_request(reqs[i]);
     if (st[i] == PKDE_OK) {
-      fp[i] = footprint_of(ctx, reqs[i], reqs[i].n);
+      fp[i] = footprint_of(ctx, reqs[i], reqs[i].n, ctx->lane_chunk_bytes);
       st[i] = fp[i].status;
     }
   }
@@ -260,7 +267,7 @@ static int estimate_batch_impl(pkde_ctx* ctx, int32_t count, const pkde_request*
     for (auto& kv : ln.plans)
       if (kv.first == keys[i]) pl = kv.second;
     if (!pl) {
-      st[i] = make_lane_plan(ctx, reqs[i], &pl);
+      st[i] = make_lane_plan(ctx, reqs[i], &pl, ctx->lane_chunk_bytes);
       if (st[i]) continue;
       ln.plans.push_back({keys[i], pl});
     }
@@ -340,7 +347,7 @@ extern "C" int pkde_gradepro_request_footprint(pkde_ctx* ctx, const pkde_request
   if (!ctx || !req || !bytes_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
   const int rc = check_request(*req);
   if (rc) return rc;
-  const Footprint f = footprint_of(ctx, *req, req->n);
+  const Footprint f = footprint_of(ctx, *req, req->n, ctx->lane_chunk_bytes);
   if (f.status) return f.status;
   *bytes_out = f.plan + f.chunk;
   return PKDE_OK;
@@ -366,6 +373,15 @@ struct Job {
 // over kMaxBypass times -- then that one goes, so no request starves.
 static constexpr int kLookahead = 16;
 static constexpr int kMaxBypass = 16;
+// A request whose ladder alone fills the GPU (G S B at least this: 256^2
+// grids in 2-D, 64^3 and up in 3-D) runs with no other such request beside
+// it; smaller requests fill the other slots.  Two GPU-filling ladders side by
+// side only share the bandwidth and their chunk buffers (60-request stream:
+// 0.986x the plain path's throughput with 4 slots of any request, 0.80x with
+// 8, 1.008x with 1).
+static constexpr double kFillWork = 1e9;
+
+static bool fills_gpu(const Job& j) { return j.fp.ladder_work >= kFillWork; }
 
 struct PoolPlan {
   pkde_plan* pl = nullptr;
@@ -405,6 +421,7 @@ struct pkde_server {
   uint64_t next_ticket = 1;
   uint64_t completed = 0, rejected = 0;
   uint64_t bytes_in_use = 0;  // pool plans + slot chunk buffers (read under mu)
+  uint64_t slot_chunk = 0;    // chunk budget of a slot's plans
   std::vector<double> lat;
   bool stop = false;
 };
@@ -458,9 +475,12 @@ static void finish_job(pkde_server* sv, Slot& sl) {
   complete(sv, job, st, false);
 }
 
-// Frees idle pool plans, least recently used first (never `keep`), until
-// `need` more bytes fit the budget.  True when they fit.
-static bool make_room(pkde_server* sv, uint64_t need, const PoolPlan* keep) {
+// Frees idle pool plans, least recently used first (never `keep`), then the
+// chunk buffers of idle slots (never `keep_slot`'s; the plans that borrowed
+// one borrow their next slot's buffer when they start), until `need` more
+// bytes fit the budget.  True when they fit.
+static bool make_room(pkde_server* sv, uint64_t need, const PoolPlan* keep,
+                      const Slot* keep_slot) {
   for (;;) {
     if (resident_bytes(sv) + need <= sv->cfg.memory_budget) return true;
     int victim = -1;
@@ -469,9 +489,19 @@ static bool make_room(pkde_server* sv, uint64_t need, const PoolPlan* keep) {
       if (p->busy || p == keep) continue;
       if (victim < 0 || p->last_used < sv->pool[victim]->last_used) victim = i;
     }
-    if (victim < 0) return false;
-    pkde_gradepro_plan_destroy(sv->pool[victim]->pl);
-    sv->pool.erase(sv->pool.begin() + victim);
+    if (victim >= 0) {
+      pkde_gradepro_plan_destroy(sv->pool[victim]->pl);
+      sv->pool.erase(sv->pool.begin() + victim);
+      continue;
+    }
+    Slot* idle = nullptr;
+    for (auto& o : sv->slots)
+      if (&o != keep_slot && !o.job && o.Y_bytes > 0 && (!idle || o.Y_bytes > idle->Y_bytes))
+        idle = &o;
+    if (!idle) return false;
+    (void)hipFree(idle->Y);
+    idle->Y = nullptr;
+    idle->Y_bytes = 0;
   }
 }
 
@@ -514,21 +544,28 @@ static int start_job(pkde_server* sv, Slot& sl, const std::shared_ptr<Job>& job)
     need += job->fp.plan;
   else if (pp->n_cap < job->n_cap)
     need += job->fp.plan > pp->bytes ? job->fp.plan - pp->bytes : 0;
-  if (!make_room(sv, need, pp)) return 0;
+  if (!make_room(sv, need, pp, &sl)) return 0;
   int st = PKDE_OK;
-  if (sl.Y_bytes < job->fp.chunk) {
-    // grow to the largest chunk buffer any slot holds, so that the slots
-    // reach their final size in few growths
+  const uint64_t plan_need = need - (sl.Y_bytes < job->fp.chunk ? job->fp.chunk - sl.Y_bytes : 0);
+  const uint64_t pref = std::max(job->fp.chunk_own, job->fp.chunk);
+  if (sl.Y_bytes < pref &&
+      resident_bytes(sv) + (pref - sl.Y_bytes) + plan_need <= sv->cfg.memory_budget) {
+    // the larger of the slot budget's chunk buffer and the one a plan of its
+    // own would take (rungs of 128^3: 16 to a chunk rather than 8), when it
+    // fits beside the resident set as it is
+    if ((st = grow_chunk(sl.Y, sl.Y_bytes, pref, "slot chunk buffer"))) return -st;
+  } else if (sl.Y_bytes < job->fp.chunk) {
+    // grow to the largest chunk buffer any slot holds (up to the slot
+    // budget's size), so that the slots reach their final size in few growths
     uint64_t want = job->fp.chunk;
-    for (const auto& o : sv->slots) want = std::max(want, o.Y_bytes);
-    if (!make_room(sv, want - sl.Y_bytes + (need - (job->fp.chunk - sl.Y_bytes)), pp))
-      want = job->fp.chunk;
+    for (const auto& o : sv->slots) want = std::max(want, std::min(o.Y_bytes, sv->slot_chunk));
+    if (!make_room(sv, want - sl.Y_bytes + plan_need, pp, &sl)) want = job->fp.chunk;
     if ((st = grow_chunk(sl.Y, sl.Y_bytes, want, "slot chunk buffer"))) return -st;
   }
   if (!pp) {
     auto np = std::make_unique<PoolPlan>();
     pkde_plan* pl = nullptr;
-    if ((st = make_lane_plan(sv->ctx, r, &pl))) return -st;
+    if ((st = make_lane_plan(sv->ctx, r, &pl, sv->slot_chunk))) return -st;
     np->pl = pl;
     np->key = job->key;
     pp = np.get();
@@ -586,16 +623,24 @@ static void server_loop(pkde_server* sv) {
       std::shared_ptr<Job> job;
       size_t pick = 0;
       {
+        bool filling = false;  // a GPU-filling request is in flight
+        for (const auto& o : sv->slots) filling = filling || (o.job && fills_gpu(*o.job));
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->queue.empty()) break;
         const size_t win = std::min<size_t>(sv->queue.size(), kLookahead);
+        bool found = false;
         for (size_t i = 0; i < win; ++i) {
-          if (sv->queue[i]->bypassed >= kMaxBypass) {
+          const Job& c = *sv->queue[i];
+          if (filling && fills_gpu(c)) continue;  // waits for the one in flight
+          if (c.bypassed >= kMaxBypass) {
             pick = i;
+            found = true;
             break;
           }
-          if (sv->queue[i]->fp.work < sv->queue[pick]->fp.work) pick = i;
+          if (!found || c.fp.work < sv->queue[pick]->fp.work) pick = i;
+          found = true;
         }
+        if (!found) break;  // only GPU-filling requests wait in the window
         job = sv->queue[pick];
       }
       auto dequeue = [&] {
@@ -650,6 +695,14 @@ int pkde_server_create(pkde_ctx* ctx, const pkde_server_cfg* cfg, pkde_server** 
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   sv->cfg.memory_budget = (cfg && cfg->memory_budget > 0) ? cfg->memory_budget : (uint64_t)(fr * 0.8);
+  // A slot's plans take chunk buffers of up to half the budget over the
+  // slots (at least the batched form's lane budget): rungs of 3.4 GB (128^3)
+  // then run 8 to a chunk rather than 4, so the spectra are read half as
+  // often, and the slots' buffers together leave half the budget to plans.
+  sv->slot_chunk = ctx->lane_chunk_bytes;
+  if (!ctx->chunk_fixed)
+    sv->slot_chunk = std::max<uint64_t>(sv->slot_chunk,
+                                        sv->cfg.memory_budget / (2 * (uint64_t)sv->cfg.max_inflight));
   sv->slots.resize(sv->cfg.max_inflight);
   for (auto& sl : sv->slots) {
     if (!rc) rc = check_hip(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "stream");
@@ -676,7 +729,7 @@ int pkde_server_submit(pkde_server* sv, const pkde_request* req, uint64_t* ticke
   // footprint on the host, for a plan reserved for the next power of two of
   // samples (plans of a shape are reused by later requests)
   job->n_cap = std::max<int64_t>(pow2_ceil(req->n), kServeMinSamples);
-  job->fp = footprint_of(sv->ctx, *req, job->n_cap);
+  job->fp = footprint_of(sv->ctx, *req, job->n_cap, sv->slot_chunk);
   if (job->fp.status) return job->fp.status;
   job->samples.assign(req->samples, req->samples + (size_t)req->n * req->ndims);
   job->req.samples = job->samples.data();
