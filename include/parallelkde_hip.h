@@ -267,11 +267,14 @@ int pkde_gradepro_request_footprint(pkde_ctx *ctx, const pkde_request *req, uint
 /* ---- serving form: continuous batching with HBM-footprint admission ----
  * A server thread admits requests as soon as a stream slot is free --
  * among the first 16 queued, the one with the least estimated device work
- * first, but none passed over more than 16 times -- and the device memory
- * the request still needs (a plan
+ * first, but none passed over more than 16 times; a request whose ladder
+ * fills the GPU by itself (2-D 256^2, 3-D 64^3 and larger) waits while
+ * another such request runs, smaller ones fill the other slots -- and the
+ * device memory the request still needs (a plan
  * for its shape, a larger chunk buffer for the slot) fits the budget next to
  * everything resident (bytes_in_use: the pooled plans and the slots' chunk
- * buffers); idle plans are evicted, least recently used first, to make room.
+ * buffers); idle plans, least recently used first, then idle slots' chunk
+ * buffers are released to make room.
  * A request whose footprint alone exceeds the budget completes with
  * PKDE_OUT_OF_MEMORY; the others go on.  Plans are pooled per grid shape and
  * keywords and reused by later requests; none is freed while a stream uses
