@@ -208,6 +208,17 @@ int plan_reserve(pkde_plan* pl, int64_t n, bool need_idx) {
   PlanBuffers& d = pl->d;
   const int64_t G = pl->g.G, F = 2 * (int64_t)pl->B + 1, fs = fsize(pl);
   const size_t rs = real_size(pl->prec);
+  if (!d.acc) {
+    // check_memory (GradeProEstimator.jl:1098-1112): the plan's whole
+    // footprint (plan_bytes, the footprint model extending get_necessary_memory) against the
+    // device's free memory, before anything is allocated
+    size_t fr = 0, tot = 0;
+    const uint64_t need = plan_bytes(pl, n, !d.Y_borrowed);
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && need > (uint64_t)fr)
+      return fail(PKDE_OUT_OF_MEMORY, "Not enough memory available. Required: " +
+                                          std::to_string(need >> 20) + " MiB, Available: " +
+                                          std::to_string((uint64_t)fr >> 20) + " MiB");
+  }
   if (n > d.x_cap) {
     dfree(d.x);
     d.x = nullptr;

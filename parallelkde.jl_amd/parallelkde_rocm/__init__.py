@@ -108,6 +108,15 @@ def context() -> _Context:
         return _ctx
 
 
+def device_free_memory():
+    """(free, total) bytes of the device's memory: get_available_memory
+    (Devices.jl:103-108) for the :rocm device."""
+    free, total = _lib.C.c_uint64(), _lib.C.c_uint64()
+    _raise(_lib.load().pkde_device_free_memory(context().handle, _lib.C.byref(free),
+                                                _lib.C.byref(total)), "free memory")
+    return int(free.value), int(total.value)
+
+
 # ------------------------------------------------------------------ grids
 @dataclass
 class Grid:

@@ -89,6 +89,27 @@ def test_server_rejects_over_budget_request_others_complete(pk):
         assert np.array_equal(d, dp) and np.array_equal(c, cp)
 
 
+def test_plain_estimate_over_device_memory_is_refused_before_allocating(pk):
+    """check_memory (GradeProEstimator.jl:1098-1112): an estimate whose
+    footprint exceeds the device's free memory (a 1024 x 1024 x 256 grid:
+    its 201 binning fields alone are ~430 GB) raises the reference's
+    ArgumentError("Not enough memory available. ...") before any device
+    allocation, and the device's free memory and later estimates are
+    untouched."""
+    free0 = pk.device_free_memory()[0]
+    grid = pk.initialize_grid([(-4.0, 4.0, 1024), (-4.0, 4.0, 1024), (-4.0, 4.0, 256)])
+    est = pk.initialize_estimation(W.correlated_3d(1000, 91), grid=grid)
+    with pytest.raises(pk.ArgumentError, match="Not enough memory available"):
+        pk.estimate_density(est, "gradepro", n_bootstraps=100)
+    est = None
+    free1 = pk.device_free_memory()[0]
+    say(f"  free device memory before {free0 >> 20} MiB, after {free1 >> 20} MiB")
+    assert free1 >= free0 - (256 << 20)
+    x, g, kw = _small_requests(pk)[0]
+    d, c = _plain(pk, x, g, **kw)
+    assert np.all(np.isfinite(d))
+
+
 def test_server_failed_request_does_not_poison_others(pk):
     """A request with a sample outside its grid fails with an ArgumentError
     (check_data, DirectSpace.jl:479-498); the requests in flight with it
