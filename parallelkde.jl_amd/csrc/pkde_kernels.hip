@@ -2395,6 +2395,62 @@ __global__ __launch_bounds__(kThreads) void k_axis_gen_rows(AxisArgs<double> a) 
   }
 }
 
+// Ladder form of k_axis_gen_rows: one block per (field, 8-column tile,
+// share of the chunk's rungs) reads its spectrum columns once into LDS and,
+// for every rung of its share, applies that rung's Gaussian factor, runs the
+// wave-local FFT and stores the tile rows -- the spectra are read once per
+// share instead of once per rung.  The arithmetic is k_axis_gen_rows's.
+__global__ __launch_bounds__(kThreads) void k_axis_gen_ladder(AxisArgs<double> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = a.L;
+  const int PL = gen_pencil_stride(L);
+  cplx<double>* buf = (cplx<double>*)smem;          // working pencils [kGenTK][PL]
+  cplx<double>* raw = buf + kGenTK * PL;             // spectrum columns [kGenTK][L]
+  cplx<double>* twl = raw + kGenTK * L;
+  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
+  uint32_t bid = blockIdx.x;
+  const int sp = (int)(bid % (uint32_t)a.nsplit);
+  bid /= (uint32_t)a.nsplit;
+  const int rbeg = sp * a.R / a.nsplit, rend = (sp + 1) * a.R / a.nsplit;
+  const int kt = (int)(bid % (uint32_t)a.ktiles);
+  const int f = a.fid0 + (int)(bid / (uint32_t)a.ktiles);
+  const int k0 = kt * kGenTK;
+  const int TKe = (a.H1 - k0) < kGenTK ? (a.H1 - k0) : kGenTK;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const cplx<double>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize + (int64_t)k0 * a.Q;
+  for (int id = tid; id < kGenTK * L; id += kThreads) {
+    const int kk = id / L, e = id - kk * L;
+    raw[id] = kk < TKe ? src[(int64_t)kk * a.Q + e] : cplx<double>{0.0, 0.0};
+  }
+  const double* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
+  for (int rr = rbeg; rr < rend; ++rr) {
+    const double* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
+    __syncthreads();  // the columns are loaded; the previous rung's stores have read buf
+    for (int id = tid; id < kGenTK * L; id += kThreads) {
+      const int kk = id / L, e = id - kk * L;
+      buf[kk * PL + pad8(e)] = kk < TKe ? cscale(raw[id], gt[e]) : cplx<double>{0.0, 0.0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int kk = 2 * wv + c;
+      if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
+    }
+    __syncthreads();
+    cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
+                         (int64_t)kt * a.Q * kGenTK;
+    for (int id = tid; id < kGenTK * L; id += kThreads) {
+      const int k = id & (kGenTK - 1);
+      const int e = id >> 3;
+      tile[(int64_t)e * kGenTK + k] = buf[k * PL + pad8(e)];
+    }
+  }
+}
+
+static size_t gen_ladder_lds(int L) {
+  return ((size_t)kGenTK * gen_pencil_stride(L) + (size_t)kGenTK * L + L) * sizeof(cplx<double>);
+}
+
 bool ladder_gen_rows_ok(const GridDev& g) {
   return (g.N == 2 || g.N == 3) && !g.pow2 && g.TKy == kGenTK && (g.n[0] + 63) / 64 <= 8 &&
          wave_gen_fits(g.facw[0]) && wave_gen_fits(g.facw[1]) &&
@@ -2558,6 +2614,21 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
   a.ktiles = g.ktY;
   a.fac = g.facw[1];
   a.twg = (const cplx<double>*)twg + g.koff[1];
+  if (gen_ladder_lds(a.L) <= 160 * 1024) {
+    // ladder form: with few tiles, split the rungs so that about 8000 blocks
+    // run, each keeping at least 8 rungs (it reads its columns once per share)
+    static const bool attr = [] {
+      (void)hipFuncSetAttribute((const void*)k_axis_gen_ladder,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      return true;
+    }();
+    (void)attr;
+    const int64_t tiles = (int64_t)nf * g.ktY;
+    a.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((kAxisSplitTarget + tiles - 1) / tiles,
+                                                           R / 8));
+    k_axis_gen_ladder<<<(unsigned)(tiles * a.nsplit), kThreads, gen_ladder_lds(a.L), s>>>(a);
+    return true;
+  }
   const int64_t blocks = (int64_t)R * nf * g.ktY;
   const size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
   k_axis_gen_rows<<<(unsigned)blocks, kThreads, lds, s>>>(a);
