@@ -359,16 +359,37 @@ __host__ __device__ constexpr int gen_pencil_stride(int L) { return L + L / 8 + 
 
 // Whether every pass of the plan f fits the wave-local mixed-radix FFT with
 // 64 lanes per pencil.
-bool wave_gen_fits(const FftFactors& f) {
+// Whether every pass of the plan f fits the wave-local mixed-radix FFT with
+// `lp` lanes per pencil.
+static bool wave_gen_fits_lp(const FftFactors& f, int lp) {
   if (f.L < 2 || f.L > 512) return false;
   int Ns = 1;
   for (int p = 0; p < f.np; ++p) {
     const int R = f.rad[p];
-    const int per_lane = (f.L / R + 63) / 64;
+    const int per_lane = (f.L / R + lp - 1) / lp;
     if (wave_gen_nb(R) == 0 || per_lane > wave_gen_nb(R)) return false;
     Ns *= R;
   }
   return Ns == f.L;
+}
+
+bool wave_gen_fits(const FftFactors& f) { return wave_gen_fits_lp(f, 64); }
+
+// A wavefront's two pencils p0 and p1 (a0, a1: whether each is in use): with
+// LP = 32 the two halves of the wave transform them together (lengths whose
+// passes fit 32 lanes, e.g. 150 = 5 5 3 2), with LP = 64 one after the
+// other.  Each butterfly does the same arithmetic either way.
+template <int LP>
+__device__ __forceinline__ void wave_fft_gen_pair(cplx<double>* p0, bool a0, cplx<double>* p1,
+                                                  bool a1, int lane, const FftFactors& fac,
+                                                  const cplx<double>* twl) {
+  if constexpr (LP == 32) {
+    const bool hi = lane >= 32;
+    wave_fft_gen<1, double, 32>(hi ? p1 : p0, lane & 31, hi ? a1 : a0, fac, twl);
+  } else {
+    if (a0) wave_fft_gen<1, double, 64>(p0, lane, true, fac, twl);
+    if (a1) wave_fft_gen<1, double, 64>(p1, lane, true, fac, twl);
+  }
 }
 
 // Per-rung axis pass for lengths that are not powers of two and fit the
@@ -2400,6 +2421,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_gen_rows(AxisArgs<double> a) 
 // for every rung of its share, applies that rung's Gaussian factor, runs the
 // wave-local FFT and stores the tile rows -- the spectra are read once per
 // share instead of once per rung.  The arithmetic is k_axis_gen_rows's.
+template <int LP>
 __global__ __launch_bounds__(kThreads) void k_axis_gen_ladder(AxisArgs<double> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int L = a.L;
@@ -2431,11 +2453,9 @@ __global__ __launch_bounds__(kThreads) void k_axis_gen_ladder(AxisArgs<double> a
       buf[kk * PL + pad8(e)] = kk < TKe ? cscale(raw[id], gt[e]) : cplx<double>{0.0, 0.0};
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int kk = 2 * wv + c;
-      if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
-    }
+    // wave wv transforms columns 2 wv and 2 wv + 1 of the tile
+    wave_fft_gen_pair<LP>(buf + 2 * wv * PL, 2 * wv < TKe, buf + (2 * wv + 1) * PL,
+                          2 * wv + 1 < TKe, lane, a.fac, twl);
     __syncthreads();
     cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
                          (int64_t)kt * a.Q * kGenTK;
@@ -2461,6 +2481,7 @@ bool ladder_gen_rows_ok(const GridDev& g) {
 // k1, 8 consecutive j3); pencils along j2 are contiguous in the spectra
 // [k1][j2 + n2 j3]; the result is stored as [k1][j2][j3], 8 consecutive j3
 // (one 128-byte run) per (k1, j2).
+template <int LP>
 __global__ __launch_bounds__(kThreads) void k_gen3_pass1(AxisArgs<double> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<double>* buf = (cplx<double>*)smem;
@@ -2488,11 +2509,8 @@ __global__ __launch_bounds__(kThreads) void k_gen3_pass1(AxisArgs<double> a) {
     buf[j * PL + pad8(e)] = v;
   }
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int j = 2 * wv + c;
-    if (j30 + j < n3) wave_fft_gen<1, double, 64>(buf + j * PL, lane, true, a.fac, twl);
-  }
+  wave_fft_gen_pair<LP>(buf + 2 * wv * PL, j30 + 2 * wv < n3, buf + (2 * wv + 1) * PL,
+                        j30 + 2 * wv + 1 < n3, lane, a.fac, twl);
   __syncthreads();
   cplx<double>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
                       (int64_t)k1 * a.Q;
@@ -2506,6 +2524,7 @@ __global__ __launch_bounds__(kThreads) void k_gen3_pass1(AxisArgs<double> a) {
 // 8-column k1 tile); pencils along j3 are contiguous in [k1][j2][j3]; the
 // result goes to the tiled row layout with rows q' = j3 + n3 j2 (whole
 // 128-byte lines: columns past H1 are zero).
+template <int LP>
 __global__ __launch_bounds__(kThreads) void k_gen3_pass2(AxisArgs<double> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<double>* buf = (cplx<double>*)smem;
@@ -2533,11 +2552,8 @@ __global__ __launch_bounds__(kThreads) void k_gen3_pass2(AxisArgs<double> a) {
     buf[kk * PL + pad8(e)] = v;
   }
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int kk = 2 * wv + c;
-    if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
-  }
+  wave_fft_gen_pair<LP>(buf + 2 * wv * PL, 2 * wv < TKe, buf + (2 * wv + 1) * PL,
+                        2 * wv + 1 < TKe, lane, a.fac, twl);
   __syncthreads();
   cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
                        (int64_t)kt * a.Q * kGenTK + j2 * L * kGenTK;
@@ -2574,7 +2590,10 @@ bool launch_gen3_passes(const GridDev& g, const cplx<double>* spec, cplx<double>
   a.twg = (const cplx<double>*)twg + g.koff[1];
   const int jtiles = (g.n[2] + kGenTK - 1) / kGenTK;
   size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
-  k_gen3_pass1<<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
+  if (wave_gen_fits_lp(a.fac, 32))
+    k_gen3_pass1<32><<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
+  else
+    k_gen3_pass1<64><<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
   // pass 2: axis 3, into the tiled rows
   a.src = y2;
   a.src_rs = rung_stride;
@@ -2586,7 +2605,10 @@ bool launch_gen3_passes(const GridDev& g, const cplx<double>* spec, cplx<double>
   a.twg = (const cplx<double>*)twg + g.koff[2];
   a.ktiles = g.ktY;
   lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
-  k_gen3_pass2<<<(unsigned)((int64_t)R * nf * g.n[1] * g.ktY), kThreads, lds, s>>>(a);
+  if (wave_gen_fits_lp(a.fac, 32))
+    k_gen3_pass2<32><<<(unsigned)((int64_t)R * nf * g.n[1] * g.ktY), kThreads, lds, s>>>(a);
+  else
+    k_gen3_pass2<64><<<(unsigned)((int64_t)R * nf * g.n[1] * g.ktY), kThreads, lds, s>>>(a);
   return true;
 }
 
@@ -2618,7 +2640,9 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
     // ladder form: with few tiles, split the rungs so that about 8000 blocks
     // run, each keeping at least 8 rungs (it reads its columns once per share)
     static const bool attr = [] {
-      (void)hipFuncSetAttribute((const void*)k_axis_gen_ladder,
+      (void)hipFuncSetAttribute((const void*)k_axis_gen_ladder<32>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipFuncSetAttribute((const void*)k_axis_gen_ladder<64>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
       return true;
     }();
@@ -2626,7 +2650,11 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
     const int64_t tiles = (int64_t)nf * g.ktY;
     a.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((kAxisSplitTarget + tiles - 1) / tiles,
                                                            R / 8));
-    k_axis_gen_ladder<<<(unsigned)(tiles * a.nsplit), kThreads, gen_ladder_lds(a.L), s>>>(a);
+    const unsigned blocks = (unsigned)(tiles * a.nsplit);
+    if (wave_gen_fits_lp(a.fac, 32))
+      k_axis_gen_ladder<32><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
+    else
+      k_axis_gen_ladder<64><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
     return true;
   }
   const int64_t blocks = (int64_t)R * nf * g.ktY;
@@ -2639,7 +2667,7 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
 // bootstrap pair form two complex pencils, transformed one after the other;
 // then VMR and Welford over b in order for its PPT points per lane.  Loads
 // run one pair ahead.  Same arithmetic as k_ladder_rows.
-template <int PPT>
+template <int PPT, int LP>
 __global__ __launch_bounds__(kThreads) void k_ladder_rows_gen(VmrArgs<double> a) {
   constexpr int IT = PPT + 1;  // half-spectrum items per lane (2 H1 <= 64 PPT + 2)
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2713,8 +2741,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows_gen(VmrArgs<double> a)
     const double inv1 = pi1;
     const double inv2 = has2 ? pi2 : 0.0;
     wave_sync();
-    wave_fft_gen<1, double, 64>(Z, lane, true, a.fac, twl);
-    wave_fft_gen<1, double, 64>(Z + PL, lane, true, a.fac, twl);
+    wave_fft_gen_pair<LP>(Z, true, Z + PL, true, lane, a.fac, twl);
     issue(b + 2 < B ? b + 2 : b);
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -2792,15 +2819,26 @@ bool launch_ladder_rows_gen(const GridDev& g, const cplx<double>* src, int64_t s
   const int wpb = kThreads / 64;
   const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
   const size_t lds = ((size_t)wpb * 2 * gen_pencil_stride(n1) + n1) * sizeof(cplx<double>);
+  // the sums and squares pencils of a row together in the two halves of the
+  // wave when the row length fits 32 lanes
+  if (wave_gen_fits_lp(a.fac, 32)) {
+    switch (ppt) {
+      case 1: k_ladder_rows_gen<1, 32><<<blocks, kThreads, lds, s>>>(a); break;
+      case 2: k_ladder_rows_gen<2, 32><<<blocks, kThreads, lds, s>>>(a); break;
+      case 3: k_ladder_rows_gen<3, 32><<<blocks, kThreads, lds, s>>>(a); break;
+      default: k_ladder_rows_gen<4, 32><<<blocks, kThreads, lds, s>>>(a); break;
+    }
+    return true;
+  }
   switch (ppt) {
-    case 1: k_ladder_rows_gen<1><<<blocks, kThreads, lds, s>>>(a); break;
-    case 2: k_ladder_rows_gen<2><<<blocks, kThreads, lds, s>>>(a); break;
-    case 3: k_ladder_rows_gen<3><<<blocks, kThreads, lds, s>>>(a); break;
-    case 4: k_ladder_rows_gen<4><<<blocks, kThreads, lds, s>>>(a); break;
-    case 5: k_ladder_rows_gen<5><<<blocks, kThreads, lds, s>>>(a); break;
-    case 6: k_ladder_rows_gen<6><<<blocks, kThreads, lds, s>>>(a); break;
-    case 7: k_ladder_rows_gen<7><<<blocks, kThreads, lds, s>>>(a); break;
-    default: k_ladder_rows_gen<8><<<blocks, kThreads, lds, s>>>(a); break;
+    case 1: k_ladder_rows_gen<1, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 2: k_ladder_rows_gen<2, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 3: k_ladder_rows_gen<3, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 4: k_ladder_rows_gen<4, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 5: k_ladder_rows_gen<5, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 6: k_ladder_rows_gen<6, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    case 7: k_ladder_rows_gen<7, 64><<<blocks, kThreads, lds, s>>>(a); break;
+    default: k_ladder_rows_gen<8, 64><<<blocks, kThreads, lds, s>>>(a); break;
   }
   return true;
 }
