@@ -379,16 +379,16 @@ bool wave_gen_fits(const FftFactors& f) { return wave_gen_fits_lp(f, 64); }
 // LP = 32 the two halves of the wave transform them together (lengths whose
 // passes fit 32 lanes, e.g. 150 = 5 5 3 2), with LP = 64 one after the
 // other.  Each butterfly does the same arithmetic either way.
-template <int LP>
-__device__ __forceinline__ void wave_fft_gen_pair(cplx<double>* p0, bool a0, cplx<double>* p1,
-                                                  bool a1, int lane, const FftFactors& fac,
-                                                  const cplx<double>* twl) {
+template <int LP, typename T = double>
+__device__ __forceinline__ void wave_fft_gen_pair(cplx<T>* p0, bool a0, cplx<T>* p1, bool a1,
+                                                  int lane, const FftFactors& fac,
+                                                  const cplx<T>* twl) {
   if constexpr (LP == 32) {
     const bool hi = lane >= 32;
-    wave_fft_gen<1, double, 32>(hi ? p1 : p0, lane & 31, hi ? a1 : a0, fac, twl);
+    wave_fft_gen<1, T, 32>(hi ? p1 : p0, lane & 31, hi ? a1 : a0, fac, twl);
   } else {
-    if (a0) wave_fft_gen<1, double, 64>(p0, lane, true, fac, twl);
-    if (a1) wave_fft_gen<1, double, 64>(p1, lane, true, fac, twl);
+    if (a0) wave_fft_gen<1, T, 64>(p0, lane, true, fac, twl);
+    if (a1) wave_fft_gen<1, T, 64>(p1, lane, true, fac, twl);
   }
 }
 
@@ -1550,7 +1550,7 @@ __global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> 
 // The same for lengths that are not powers of two but fit the wave-local
 // mixed-radix FFT (a.fac: radices 2, 3, 4, 5, 8; n1 <= 512): PPT = points per
 // thread of the Welford phase.
-template <typename T>
+template <typename T, int LP>
 __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d_gen(VmrArgs<T> a) {
   constexpr int NT = 64 * kWaves1d;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1608,8 +1608,7 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d_gen(VmrArgs<T> a) {
         }
       }
       wave_sync();
-      wave_fft_gen<1, T, 64>(Z, lane, true, a.fac, twl);
-      wave_fft_gen<1, T, 64>(Z + PL, lane, true, a.fac, twl);
+      wave_fft_gen_pair<LP, T>(Z, true, Z + PL, true, lane, a.fac, twl);
       for (int jj = lane; jj < n1; jj += 64) {
         const cplx<T> zm = Z[pad8(jj)];
         const cplx<T> zv = Z[PL + pad8(jj)];
@@ -1657,16 +1656,22 @@ static bool ladder_1d_gen_launch(const GridDev& g, unsigned blocks, hipStream_t 
   const size_t lds = ((size_t)kWaves1d * 2 * gen_pencil_stride(n1) + n1) * sizeof(cplx<T>);
   static const bool attr = [] {
     // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
-    (void)hipFuncSetAttribute((const void*)k_ladder_1d_gen<T>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(((size_t)kWaves1d * 2 * gen_pencil_stride(512) + 512) *
-                                    sizeof(cplx<T>)));
+    const int most = (int)(((size_t)kWaves1d * 2 * gen_pencil_stride(512) + 512) * sizeof(cplx<T>));
+    (void)hipFuncSetAttribute((const void*)k_ladder_1d_gen<T, 32>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, most);
+    (void)hipFuncSetAttribute((const void*)k_ladder_1d_gen<T, 64>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, most);
     return true;
   }();
   (void)attr;
   VmrArgs<T> b = a;
   b.fac = g.facw[0];
-  k_ladder_1d_gen<T><<<blocks, 64 * kWaves1d, lds, s>>>(b);
+  // the sums and squares pencils together in the two halves of the wave
+  // when the length fits 32 lanes
+  if (wave_gen_fits_lp(b.fac, 32))
+    k_ladder_1d_gen<T, 32><<<blocks, 64 * kWaves1d, lds, s>>>(b);
+  else
+    k_ladder_1d_gen<T, 64><<<blocks, 64 * kWaves1d, lds, s>>>(b);
   return true;
 }
 
