@@ -2482,46 +2482,56 @@ bool ladder_gen_rows_ok(const GridDev& g) {
          (g.N == 2 || wave_gen_fits(g.facw[2]));
 }
 
-// 3-D, first pass (grid axis 2, length n2 = a.I): one block per (rung, field,
-// k1, 8 consecutive j3); pencils along j2 are contiguous in the spectra
-// [k1][j2 + n2 j3]; the result is stored as [k1][j2][j3], 8 consecutive j3
-// (one 128-byte run) per (k1, j2).
+// 3-D, first pass (grid axis 2, length n2 = a.I), ladder form: one block per
+// (field, k1, 8 consecutive j3, share of the chunk's rungs) reads its 8
+// spectrum pencils along j2 (contiguous in [k1][j2 + n2 j3]) once into LDS
+// and, for every rung of its share, applies the rung's axis-2 factor, runs
+// the wave-local FFT and stores [k1][j2][j3], 8 consecutive j3 (one 128-byte
+// run) per (k1, j2).
 template <int LP>
 __global__ __launch_bounds__(kThreads) void k_gen3_pass1(AxisArgs<double> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<double>* buf = (cplx<double>*)smem;
   const int L = a.L;  // n2
   const int PL = gen_pencil_stride(L);
-  cplx<double>* twl = buf + kGenTK * PL;
+  cplx<double>* buf = (cplx<double>*)smem;  // working pencils [kGenTK][PL]
+  cplx<double>* raw = buf + kGenTK * PL;     // spectrum pencils [kGenTK][L]
+  cplx<double>* twl = raw + kGenTK * L;
   for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
   const int64_t n3 = a.O;
   const int jtiles = (int)((n3 + kGenTK - 1) / kGenTK);
   uint32_t bid = blockIdx.x;
+  const int sp = (int)(bid % (uint32_t)a.nsplit);
+  bid /= (uint32_t)a.nsplit;
+  const int rbeg = sp * a.R / a.nsplit, rend = (sp + 1) * a.R / a.nsplit;
   const int jt = (int)(bid % (uint32_t)jtiles);
   bid /= (uint32_t)jtiles;
   const int k1 = (int)(bid % (uint32_t)a.H1);
-  bid /= (uint32_t)a.H1;
-  const int f = a.fid0 + (int)(bid % (uint32_t)a.nf);
-  const int rr = (int)(bid / (uint32_t)a.nf);
+  const int f = a.fid0 + (int)(bid / (uint32_t)a.H1);
   const int j30 = jt * kGenTK;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const double* gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
   const cplx<double>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize + (int64_t)k1 * a.Q;
   for (int id = tid; id < kGenTK * L; id += kThreads) {
     const int j = id / L, e = id - j * L;  // pencil j: j3 = j30 + j
-    cplx<double> v = {0.0, 0.0};
-    if (j30 + j < n3) v = cscale(src[(int64_t)(j30 + j) * L + e], gt[e]);
-    buf[j * PL + pad8(e)] = v;
+    raw[id] = j30 + j < n3 ? src[(int64_t)(j30 + j) * L + e] : cplx<double>{0.0, 0.0};
   }
-  __syncthreads();
-  wave_fft_gen_pair<LP>(buf + 2 * wv * PL, j30 + 2 * wv < n3, buf + (2 * wv + 1) * PL,
-                        j30 + 2 * wv + 1 < n3, lane, a.fac, twl);
-  __syncthreads();
-  cplx<double>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
-                      (int64_t)k1 * a.Q;
-  for (int id = tid; id < kGenTK * L; id += kThreads) {
-    const int j = id & (kGenTK - 1), e = id >> 3;
-    if (j30 + j < n3) dst[(int64_t)e * n3 + j30 + j] = buf[j * PL + pad8(e)];
+  const double* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
+  for (int rr = rbeg; rr < rend; ++rr) {
+    const double* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
+    __syncthreads();  // the pencils are loaded; the previous rung's stores have read buf
+    for (int id = tid; id < kGenTK * L; id += kThreads) {
+      const int j = id / L, e = id - j * L;
+      buf[j * PL + pad8(e)] = j30 + j < n3 ? cscale(raw[id], gt[e]) : cplx<double>{0.0, 0.0};
+    }
+    __syncthreads();
+    wave_fft_gen_pair<LP>(buf + 2 * wv * PL, j30 + 2 * wv < n3, buf + (2 * wv + 1) * PL,
+                          j30 + 2 * wv + 1 < n3, lane, a.fac, twl);
+    __syncthreads();
+    cplx<double>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
+                        (int64_t)k1 * a.Q;
+    for (int id = tid; id < kGenTK * L; id += kThreads) {
+      const int j = id & (kGenTK - 1), e = id >> 3;
+      if (j30 + j < n3) dst[(int64_t)e * n3 + j30 + j] = buf[j * PL + pad8(e)];
+    }
   }
 }
 
@@ -2594,11 +2604,27 @@ bool launch_gen3_passes(const GridDev& g, const cplx<double>* spec, cplx<double>
   a.fac = g.facw[1];
   a.twg = (const cplx<double>*)twg + g.koff[1];
   const int jtiles = (g.n[2] + kGenTK - 1) / kGenTK;
-  size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
-  if (wave_gen_fits_lp(a.fac, 32))
-    k_gen3_pass1<32><<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
-  else
-    k_gen3_pass1<64><<<(unsigned)((int64_t)R * nf * g.H1 * jtiles), kThreads, lds, s>>>(a);
+  {
+    static const bool attr = [] {
+      (void)hipFuncSetAttribute((const void*)k_gen3_pass1<32>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipFuncSetAttribute((const void*)k_gen3_pass1<64>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      return true;
+    }();
+    (void)attr;
+    // with few tiles, split the rungs so that about 8000 blocks run, each
+    // keeping at least 8 rungs (it reads its pencils once per share)
+    const int64_t tiles = (int64_t)nf * g.H1 * jtiles;
+    a.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((kAxisSplitTarget + tiles - 1) / tiles,
+                                                           R / 8));
+    const unsigned blocks = (unsigned)(tiles * a.nsplit);
+    if (wave_gen_fits_lp(a.fac, 32))
+      k_gen3_pass1<32><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
+    else
+      k_gen3_pass1<64><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
+  }
+  size_t lds;
   // pass 2: axis 3, into the tiled rows
   a.src = y2;
   a.src_rs = rung_stride;
