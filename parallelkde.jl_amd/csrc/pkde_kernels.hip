@@ -2375,57 +2375,11 @@ template bool launch_ladder_rows<double>(const GridDev&, const cplx<double>*, in
 // wavefront, with the plan's twiddle tables staged in LDS.
 constexpr int kGenTK = 8;  // tile width of the tiled row layout (FP64)
 
-// One block per (rung, field, 8-column tile): load the tile's spectrum
-// columns along grid axis 2 with the rung's Gaussian factor, transform, store
-// the tile rows.  Columns past H1 up to the tile width are zero.
-__global__ __launch_bounds__(kThreads) void k_axis_gen_rows(AxisArgs<double> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  cplx<double>* buf = (cplx<double>*)smem;
-  const int L = a.L;
-  const int PL = gen_pencil_stride(L);
-  cplx<double>* twl = buf + kGenTK * PL;
-  for (int k = threadIdx.x; k < L; k += kThreads) twl[k] = a.twg[k];
-  uint32_t bid = blockIdx.x;
-  const int kt = (int)(bid % (uint32_t)a.ktiles);
-  bid /= (uint32_t)a.ktiles;
-  const int f = a.fid0 + (int)(bid % (uint32_t)a.nf);
-  const int rr = (int)(bid / (uint32_t)a.nf);
-  const int k0 = kt * kGenTK;
-  const int TKe = (a.H1 - k0) < kGenTK ? (a.H1 - k0) : kGenTK;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const double* gt = (f < a.fsplit ? a.gM : a.gV) + (int64_t)(a.r0 + rr) * a.ksum + a.goff;
-  // wave wv owns columns 2 wv and 2 wv + 1 of the tile
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int kk = 2 * wv + c;
-    const bool act = kk < TKe;
-    const cplx<double>* col =
-        a.src + (int64_t)(f - a.src_fid0) * a.fsize + (int64_t)(k0 + (act ? kk : 0)) * a.Q;
-    cplx<double>* pb = buf + kk * PL;
-    for (int e = lane; e < L; e += 64)
-      pb[pad8(e)] = act ? cscale(col[e], gt[e]) : cplx<double>{0.0, 0.0};
-  }
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int kk = 2 * wv + c;
-    if (kk < TKe) wave_fft_gen<1, double, 64>(buf + kk * PL, lane, true, a.fac, twl);
-  }
-  __syncthreads();
-  cplx<double>* tile = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize +
-                       (int64_t)kt * a.Q * kGenTK;
-  for (int id = tid; id < kGenTK * L; id += kThreads) {
-    const int k = id & (kGenTK - 1);
-    const int e = id >> 3;
-    tile[(int64_t)e * kGenTK + k] = buf[k * PL + pad8(e)];
-  }
-}
-
-// Ladder form of k_axis_gen_rows: one block per (field, 8-column tile,
-// share of the chunk's rungs) reads its spectrum columns once into LDS and,
-// for every rung of its share, applies that rung's Gaussian factor, runs the
-// wave-local FFT and stores the tile rows -- the spectra are read once per
-// share instead of once per rung.  The arithmetic is k_axis_gen_rows's.
+// Axis-2 pass of the mixed-radix 2-D ladder: one block per (field, 8-column
+// tile, share of the chunk's rungs) reads its spectrum columns once into LDS
+// and, for every rung of its share, applies that rung's Gaussian factor, runs
+// the wave-local FFT and stores the tile's rows into the tiled row layout
+// (whole 128-byte lines: columns past H1 up to the tile width are zero).
 template <int LP>
 __global__ __launch_bounds__(kThreads) void k_axis_gen_ladder(AxisArgs<double> a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2667,9 +2621,10 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
   a.ktiles = g.ktY;
   a.fac = g.facw[1];
   a.twg = (const cplx<double>*)twg + g.koff[1];
-  if (gen_ladder_lds(a.L) <= 160 * 1024) {
-    // ladder form: with few tiles, split the rungs so that about 8000 blocks
-    // run, each keeping at least 8 rungs (it reads its columns once per share)
+  {
+    // with few tiles, split the rungs so that about 8000 blocks run, each
+    // keeping at least 8 rungs (it reads its columns once per share); every
+    // length that fits the wave-local FFT (<= 512) fits LDS
     static const bool attr = [] {
       (void)hipFuncSetAttribute((const void*)k_axis_gen_ladder<32>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -2686,11 +2641,7 @@ bool launch_axis_gen_rows(const GridDev& g, const cplx<double>* spec, cplx<doubl
       k_axis_gen_ladder<32><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
     else
       k_axis_gen_ladder<64><<<blocks, kThreads, gen_ladder_lds(a.L), s>>>(a);
-    return true;
   }
-  const int64_t blocks = (int64_t)R * nf * g.ktY;
-  const size_t lds = ((size_t)kGenTK * gen_pencil_stride(a.L) + a.L) * sizeof(cplx<double>);
-  k_axis_gen_rows<<<(unsigned)blocks, kThreads, lds, s>>>(a);
   return true;
 }
 
