@@ -430,7 +430,7 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       // 3-D: [k1][j2][k3] for the axis-3 pass and the whole slab kernel; the
       // first radix-4 step of axis 3 on store for the slab kernel in quarters
       const int mode1 = N == 2 ? (rows ? 1 : 0)
-                               : (rows3 ? (slab && !slab3_ok(g) ? kSlabSplitMode : 2) : 0);
+                               : (rows3 ? (slab && !slab3_ok(g) ? 3 : 2) : 0);
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s,
                                  pl->tune.split)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");

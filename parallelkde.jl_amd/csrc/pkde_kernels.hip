@@ -636,18 +636,16 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // of the row kernel (2-D; the block's tile is one contiguous run), 2
 // [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass), 3 the first
 // radix-4 decimation-in-frequency step of the axis-3 inverse transform applied
-// on store: the block's 16 pencils are k3 = k' + s n3/4 for 4 consecutive k'
-// and s = 0..3 (each scaled by its axis-3 factor g3(k3) with the axis-2
-// factors), and it stores, for each j2 and residue class p = j3 mod 4,
-// c_p(k') = w^(p k') sum_s i^(p s) Y(k' + s n3/4), w =
-// exp(2 pi i / n3), at [k1][j2][k' / 4][p][k' mod 4] -- one contiguous
-// 256-byte run per (k1, j2); the n3/4-point inverse transform of c_p over k'
-// gives the outputs j3 = 4 m + p (k_slab3 with SPLIT = 4).  4: the same step
-// with 32 pencils per block (8 consecutive k', NT = 512 threads), stored
-// class-major, c_p(k') at [k1][j2][p][k']: each class's values of a (k1, j2)
-// are one 512-byte run, written by the blocks in whole 128-byte lines (8
-// values of k') and read by the quarter's workgroup in whole lines.  The
-// arithmetic is that of k_axis_wave with DIR = +1.
+// on store (NT = 512 threads): the block's 32 pencils are k3 = k' + s n3/4
+// for 8 consecutive k' and s = 0..3 (each scaled by its axis-3 factor g3(k3)
+// with the axis-2 factors), and it stores, for each j2 and residue class
+// p = j3 mod 4, c_p(k') = w^(p k') sum_s i^(p s) Y(k' + s n3/4), w =
+// exp(2 pi i / n3), class-major at [k1][j2][p][k']: each class's values of a
+// (k1, j2) are one contiguous run, written by the blocks in whole 128-byte
+// lines (8 values of k') and read by the class's workgroup in whole lines;
+// the n3/4-point inverse transform of c_p over k' gives the outputs
+// j3 = 4 m + p (k_slab3 with SPLIT = 4).  The arithmetic is that of
+// k_axis_wave with DIR = +1.
 template <typename T, int LOG2L, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
@@ -668,7 +666,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
   bid /= (uint32_t)a.nsplit;
   const int rbeg = sp * a.R / a.nsplit, rend = (sp + 1) * a.R / a.nsplit;
   uint32_t kt, ot;
-  if (a.mode >= 3) {
+  if (a.mode == 3) {
     // outer tiles fastest: blocks that run together fill one k1 column
     ot = bid & ((1u << a.log2otiles) - 1u);
     bid >>= a.log2otiles;
@@ -693,10 +691,10 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
   const bool act = kk < TKe && p < TK * TJ;
   const cplx<T>* src = a.src + (int64_t)(f - a.src_fid0) * a.fsize;
   const T* gbase = (f < a.fsplit ? a.gM : a.gV) + a.goff;
-  // the pencil's outer index: o0 + jj, or in modes 3 and 4 k3 = k' + s n3/4
-  // with k' = Cq ot + (jj mod Cq), s = jj / Cq, Cq = TJ / 4 values of k'
+  // the pencil's outer index: o0 + jj, or in mode 3 k3 = k' + s n3/4 with
+  // k' = Cq ot + (jj mod Cq), s = jj / Cq, Cq = TJ / 4 values of k'
   const int log2Cq = log2TJ - 2, Cq = 1 << log2Cq;
-  const int64_t oq = a.mode >= 3 ? (int64_t)ot * Cq + (jj & (Cq - 1)) +
+  const int64_t oq = a.mode == 3 ? (int64_t)ot * Cq + (jj & (Cq - 1)) +
                                        (int64_t)(jj >> log2Cq) * (a.O >> 2)
                                  : o0 + jj;
   cplx<T> x[E];
@@ -729,22 +727,12 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
     for (int id = tid; id < ((TKw - TKe) << LOG2L); id += NT)
       buf[(TKe + (id >> LOG2L)) * PL + pad8(id & (L - 1))] = {(T)0, (T)0};
   }
-  // modes 3 and 4: a thread's entries of the store all have the same
-  // j = tid mod TJ (class p = j / Cq, k' = Cq ot + j mod Cq; NT is a multiple
-  // of TJ), so its twiddle exp(+2 pi i p k' / n3) is fixed
-  cplx<T> w3 = {(T)1, (T)0};
-  if (a.mode >= 3) {
-    const int jt = tid & (TJ - 1);
-    const int pc = jt >> log2Cq, kq = (int)ot * Cq + (jt & (Cq - 1));
-    const cplx<T> t3 = a.tw[(a.O - 2) + ((pc * kq) & (int)(a.O - 1))];  // exp(-2 pi i k / n3)
-    w3 = {t3.re, -t3.im};
-  }
 
   for (int rr = rbeg; rr < rend; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
-    // modes 3 and 4: the pencil's axis-3 factor (k3 = oq is fixed along it)
-    // goes into the first pass with the axis-2 factors
-    const T g3 = a.mode >= 3 ? ((f < a.fsplit ? a.gM : a.gV) + a.goff3 +
+    // mode 3: the pencil's axis-3 factor (k3 = oq is fixed along it) goes
+    // into the first pass with the axis-2 factors
+    const T g3 = a.mode == 3 ? ((f < a.fsplit ? a.gM : a.gV) + a.goff3 +
                                 (int64_t)(a.r0 + rr) * a.ksum)[act ? oq : 0]
                              : (T)1;
     if (act) {
@@ -754,7 +742,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
         cplx<T> w[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r)
-          w[r] = cscale(x[u * R1 + r], a.mode >= 3 ? gt[j + r * nb1] * g3 : gt[j + r * nb1]);
+          w[r] = cscale(x[u * R1 + r], a.mode == 3 ? gt[j + r * nb1] * g3 : gt[j + r * nb1]);
         dft_reg<R1, 1, T>(w);
 #pragma unroll
         for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
@@ -785,7 +773,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
           chunk_store(dst + (int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + o0 + j,
                       buf[((k << log2TJ) + j) * PL + pad8(e)]);
       }
-    } else if (a.mode == 4) {
+    } else if (a.mode == 3) {
       // class-major radix-4 step: a thread takes one (k1, k' = Cq ot + c, j2),
       // reads the four values Y(k' + s n3/4) once and stores all four classes
       // at [p][k']; 8 lanes write the 8 values of k' of one class, so every
@@ -820,30 +808,6 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
           }
         }
       }
-    } else if (a.mode == 3) {
-      // radix-4 step of the axis-3 inverse (TJ == 16): entry j = 4 p + c of
-      // the run is class p of k' = 4 ot + c (the axis-3 factors are in)
-      const int j = tid & (TJ - 1), pc = j >> log2Cq, c = j & (Cq - 1);
-      const int64_t jo = (int64_t)ot * 16 + j;
-      for (int id = tid; id < total; id += NT) {
-        const int e = (id >> log2TJ) & (L - 1);
-        const int k = id >> (log2TJ + LOG2L);
-        if (k < TKe) {
-          cplx<T> acc = {(T)0, (T)0};
-#pragma unroll
-          for (int sg = 0; sg < 4; ++sg) {
-            const cplx<T> y = buf[((k << log2TJ) + c + Cq * sg) * PL + pad8(e)];
-            switch ((pc * sg) & 3) {  // times i^(p s)
-              case 0: acc = {acc.re + y.re, acc.im + y.im}; break;
-              case 1: acc = {acc.re - y.im, acc.im + y.re}; break;
-              case 2: acc = {acc.re - y.re, acc.im - y.im}; break;
-              default: acc = {acc.re + y.im, acc.im - y.re}; break;
-            }
-          }
-          const cplx<T> out = {acc.re * w3.re - acc.im * w3.im, acc.re * w3.im + acc.im * w3.re};
-          chunk_store(dst + (int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + jo, out);
-        }
-      }
     } else {
       for (int id = tid; id < total; id += NT) {
         const int e = id & (L - 1);
@@ -874,7 +838,7 @@ template <typename T>
 static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
                                const AxisArgs<T>& a) {
   if constexpr (sizeof(T) == 8) {
-    if (a.mode == 4) {
+    if (a.mode == 3) {
       // 32 pencils per block, 512 threads, 72 KB of LDS (above the 64 KB
       // default: the attribute is set once per instantiation)
       if (lds > 96 * 1024) return false;
@@ -954,20 +918,18 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.TI = 1;
   a.mode = mode;
   a.goff3 = g.N == 3 ? g.koff[2] : 0;
-  const int nt = mode == 4 ? 512 : kThreads;  // threads per block
-  if (mode == 4 && (sizeof(T) != 8 || a.L < 16 || a.L > 128)) return false;
+  const int nt = mode == 3 ? 512 : kThreads;  // threads per block
+  if (mode == 3 && (sizeof(T) != 8 || a.L < 16 || a.L > 128)) return false;
   const int per_block = (nt * Ef) / a.L;
   int hp = 1;
   while (hp < a.H1) hp <<= 1;
   // mode 2: TJ pencils along j3 give TJ-value contiguous runs of the output;
   // 16 (256-byte runs) measured 0.7 % faster than 8 at 64^3 and 128^3, 32 slower
-  a.TJ = (mode == 2 || mode == 3) ? (int)std::min<int64_t>(a.O, std::min(16, per_block))
-         : mode == 4              ? (int)std::min<int64_t>(a.O, std::min(32, per_block))
-                                  : 1;
-  // mode 3: 16 pencils (4 values of k' times the 4 quarters of axis 3);
-  // mode 4: 32 (8 values of k')
-  if (mode == 3 && (a.TJ != 16 || g.N != 3 || a.O < 16 || (a.O & 15))) return false;
-  if (mode == 4 && (a.TJ != 32 || g.N != 3 || a.O < 32 || (a.O & 31))) return false;
+  a.TJ = mode == 2   ? (int)std::min<int64_t>(a.O, std::min(16, per_block))
+         : mode == 3 ? (int)std::min<int64_t>(a.O, std::min(32, per_block))
+                     : 1;
+  // mode 3: 32 pencils (8 values of k' times the 4 quarters of axis 3)
+  if (mode == 3 && (a.TJ != 32 || g.N != 3 || a.O < 32 || (a.O & 31))) return false;
   a.TK = std::min(hp, per_block / a.TJ);
   if (mode == 1 && a.TK != g.TKy) return false;  // tile width fixed by the layout
   a.ktiles = (a.H1 + a.TK - 1) / a.TK;
@@ -1262,8 +1224,6 @@ struct VmrArgs {
   int log2TKy, rows3, log2n2, log2n3;
   int n2r, n3r;  // 3-D row remap for lengths that are not powers of two
   int goff3;     // offset of grid axis 3 in the per-rung Gaussian tables
-  int xcd;       // k_slab3 SPLIT = 4: the 4 classes of a column on one XCD
-  int cmajor;    // k_slab3 SPLIT = 4: class-major chunk ([p][k'] per (k1, j2))
   int64_t tstride;
   const double* invk;  // 1 / (b + 1), b < B (Welford weights)
   FftFactors fac;      // axis-1 plan and twiddle table (GEN kernels)
@@ -2131,7 +2091,7 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 //
 // SPLIT = 4: the axis-2 pass stored the first radix-4 step of the axis-3
 // inverse (k_axis_ladder mode 3: its axis-3 factors applied, class p of
-// k' at [k1][j2][k' / 4][p][k' mod 4]); a workgroup per (rung, j2, p) takes
+// k' at [k1][j2][p][k']); a workgroup per (rung, j2, p) takes
 // the n3 = (axis-3 length) / 4 values of class p of every pencil, runs the
 // rest of the axis-3 transform on them and writes the rows j3 = 4 m + p.  So
 // a grid whose (k1, k3) slab pair is 4x too large for LDS (128^3: 295 KB)
@@ -2181,17 +2141,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     rr = (int)blockIdx.x / n2;
     j2 = (int)blockIdx.x - rr * n2;
   } else {
-    // the 4 classes of one (rung, j2) on one XCD (workgroups go to the 8 XCDs
-    // in turn), so that their reads of the column's 256-byte runs share its L2
+    // the 4 classes of one (rung, j2) in consecutive workgroups
     const uint32_t bid = blockIdx.x;
-    uint32_t colid;
-    if (a.xcd) {
-      pcls = (int)((bid >> 3) & 3u);
-      colid = ((bid >> 5) << 3) | (bid & 7u);
-    } else {
-      pcls = (int)(bid & 3u);
-      colid = bid >> 2;
-    }
+    pcls = (int)(bid & 3u);
+    const uint32_t colid = bid >> 2;
     rr = (int)colid / n2;
     j2 = (int)colid - rr * n2;
   }
@@ -2205,13 +2158,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   const int pen = wv * ppc3 + lane / lp3, t3 = lane % lp3;
   const bool act3 = pen < 2 * H;
   const int pl = act3 ? pen / H : 0, k1p = act3 ? pen % H : 0;
-  // input i of this lane: k3 (or k' in class pcls) = t3 + i nb3; the split
-  // layouts hold k' at [k'/4][p][k' mod 4] or, class-major, at [p][k']
-  const bool cm = SPLIT == 1 || a.cmajor;
-  const int istride = cm ? nb3 : 16;
+  // input i of this lane: k3 (or k' in class pcls, stored class-major at
+  // [p][k']) = t3 + i nb3
   const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * (n3 * SPLIT) +
-                            (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q +
-                            (SPLIT == 1 ? t3 : (a.cmajor ? pcls * n3 : pcls * 4) + t3);
+                            (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q + pcls * n3 + t3;
   cplx<double>* pb3 = P + (act3 ? pen : 0) * PL3;
   double sf[E];
 #pragma unroll
@@ -2223,7 +2173,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   auto load = [&](cplx<double>(&dst)[E], int bb) {
     if (act3) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) dst[i] = chunk_load(src + (int64_t)bb * a.fsize + i * istride);
+      for (int i = 0; i < E; ++i) dst[i] = chunk_load(src + (int64_t)bb * a.fsize + i * nb3);
     }
   };
   // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
@@ -2409,8 +2359,6 @@ bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_st
   a.n2r = g.n[1];
   if (split) {
     const int64_t cols = (int64_t)R * g.n[1];
-    a.xcd = cols % 8 == 0 ? 1 : 0;
-    a.cmajor = kSlabSplitMode == 4 ? 1 : 0;
     const unsigned blocks = (unsigned)(cols * 4);
     switch (g.log2n[0]) {
       case 5: slab3_launch<5, 5, 4>(blocks, s, a); break;

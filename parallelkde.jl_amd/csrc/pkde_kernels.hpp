@@ -148,12 +148,8 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 bool slab3_ok(const GridDev& g);
 // Grids the split form takes (an axis-3 length of 128 whose whole slab does
 // not fit LDS): the axis-2 pass stores the first radix-4 step of axis 3
-// (k_axis_ladder mode kSlabSplitMode), launch_slab3 runs quarters.  Mode 4
-// (class-major, 32 pencils per block) writes and reads whole 128-byte lines;
-// mode 3 (16 pencils, 256-byte runs shared by the quarters) is kept as the
-// form the kernels also take.
+// (k_axis_ladder mode 3, class-major), launch_slab3 runs quarters.
 bool slab3_split_ok(const GridDev& g);
-constexpr int kSlabSplitMode = 4;
 bool launch_slab3(const GridDev& g, const cplx<double>* src, int64_t src_rung_stride, int B,
                   int64_t n_samples, int R, int r0, const double* gM, const double* gV,
                   const double* cV, const double* vscale, const double* tw, const double* invk,
