@@ -431,8 +431,9 @@ static int enqueue_typed(pkde_plan* pl, const double* d_x, int64_t n, const int3
       // first radix-4 step of axis 3 on store for the slab kernel in quarters
       const int mode1 = N == 2 ? (rows ? 1 : 0)
                                : (rows3 ? (slab && !slab3_ok(g) ? 3 : 2) : 0);
+      // the slab kernel takes its axis-3 and axis-1 factors from this pass
       if (!launch_axis_ladder<T>(g, spec, Y1, per_rung, 2 * B, R, r0, gM, gV, B, tw, mode1, s,
-                                 pl->tune.split)) {
+                                 pl->tune.split, slab)) {
         if (mode1) return fail(PKDE_DEVICE_ERROR, "ladder pass layout mismatch");
         launch_axis_fft<T, T, 1>(g, 1, spec, 0, 0, Y1, per_rung, 0, 0, 2 * B, R, r0, gM, gV, B, tw,
                                  twg, s);

@@ -347,7 +347,11 @@ struct AxisArgs {
   // k_axis_ladder); pack: column 0 carries X(k1 = 0) + i X(k1 = H1 columns),
   // H1 = n1 / 2
   int TJ, log2TJ, log2otiles, mode, pack;
-  int goff3;  // mode 3: offset of grid axis 3 in the per-rung Gaussian tables
+  int goff3;  // offset of grid axis 3 in the per-rung Gaussian tables
+  // 3-D input of k_slab3 (modes 2 and 3): the pencil's axis-3 factor g3(k3)
+  // and, except on the packed column 0, its axis-1 factor g1(k1) (both fixed
+  // along the pencil) go into the first pass with the axis-2 factors
+  int fold;
   int nsplit;  // ladder pass: the chunk's rungs are split over nsplit blocks per tile
   const cplx<T>* tw;
   // mixed-radix plan and twiddle table of the axis (GEN kernels)
@@ -730,11 +734,15 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
 
   for (int rr = rbeg; rr < rend; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
-    // mode 3: the pencil's axis-3 factor (k3 = oq is fixed along it) goes
-    // into the first pass with the axis-2 factors
-    const T g3 = a.mode == 3 ? ((f < a.fsplit ? a.gM : a.gV) + a.goff3 +
-                                (int64_t)(a.r0 + rr) * a.ksum)[act ? oq : 0]
-                             : (T)1;
+    // fold: the pencil's axis-3 factor (k3 = oq) times its axis-1 factor
+    // (k1 = k0 + kk; the axis-1 tables are at offset 0), one scalar per rung
+    T g13 = (T)1;
+    if (a.fold) {
+      const T* gr = gt - a.goff;
+      const int kc = act ? k0 + kk : 0;
+      const T g3 = gr[a.goff3 + (act ? oq : 0)];
+      g13 = a.pack && kc == 0 ? g3 : g3 * gr[kc];
+    }
     if (act) {
 #pragma unroll
       for (int u = 0; u < E / R1; ++u) {
@@ -742,7 +750,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
         cplx<T> w[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r)
-          w[r] = cscale(x[u * R1 + r], a.mode == 3 ? gt[j + r * nb1] * g3 : gt[j + r * nb1]);
+          w[r] = cscale(x[u * R1 + r], a.fold ? gt[j + r * nb1] * g13 : gt[j + r * nb1]);
         dft_reg<R1, 1, T>(w);
 #pragma unroll
         for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
@@ -883,7 +891,7 @@ template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
                         const T* gV, int fsplit, const T* tw, int mode, hipStream_t s,
-                        int split_target) {
+                        int split_target, bool fold) {
   AxisArgs<T> a{};
   const int Ef = sizeof(T) == 8 ? 8 : 16;
   a.L = g.n[1];
@@ -918,6 +926,9 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
   a.TI = 1;
   a.mode = mode;
   a.goff3 = g.N == 3 ? g.koff[2] : 0;
+  // the class-major split store (mode 3) feeds only k_slab3
+  a.fold = (fold || mode == 3) && g.N == 3 && (mode == 2 || mode == 3) ? 1 : 0;
+  if (mode == 3 && !a.fold) return false;
   const int nt = mode == 3 ? 512 : kThreads;  // threads per block
   if (mode == 3 && (sizeof(T) != 8 || a.L < 16 || a.L > 128)) return false;
   const int per_block = (nt * Ef) / a.L;
@@ -1024,10 +1035,10 @@ template bool launch_axis3_rows<float>(const GridDev&, const cplx<float>*, int64
                                        const float*, hipStream_t);
 template bool launch_axis_ladder<double>(const GridDev&, const cplx<double>*, cplx<double>*,
                                          int64_t, int, int, int, const double*, const double*,
-                                         int, const double*, int, hipStream_t, int);
+                                         int, const double*, int, hipStream_t, int, bool);
 template bool launch_axis_ladder<float>(const GridDev&, const cplx<float>*, cplx<float>*,
                                         int64_t, int, int, int, const float*, const float*, int,
-                                        const float*, int, hipStream_t, int);
+                                        const float*, int, hipStream_t, int, bool);
 
 template <typename T, int DIR>
 static void axis_wave_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
@@ -2072,18 +2083,19 @@ bool launch_ladder_rows(const GridDev& g, const cplx<T>* src, int64_t src_rung_s
 // a field pair fits LDS (n1 n3 up to 64 x 64, 128 x 32 or 256 x 16 in FP64):
 // one workgroup of 8 waves per (rung, j2) walks the bootstraps in order.
 // Per bootstrap b, for column j2 of its sums and squares fields ([k1][j2][k3]
-// from the axis-2 pass, k3 contiguous):
+// from the axis-2 pass, k3 contiguous, with the axis-3 and axis-1 Gaussian
+// factors already applied by that pass, except the axis-1 factors of the
+// packed column 0):
 //  - axis 3: the lanes of the pencil (plane, k1) load exactly the inputs of
-//    their first radix-8 pass, one bootstrap ahead, scale them by the axis-3
-//    factors and run that pass from registers; the other passes go through
-//    the pencil in LDS;
+//    their first radix-8 pass, one bootstrap ahead, and run that pass from
+//    registers; the other passes go through the pencil in LDS;
 //  - axis 1: the lanes of row j3 read, per input of their first radix-8
 //    pass, the pair (sums, squares) at the column that input comes from and
 //    form the row with the sums field in the real part and the squares
-//    field in the imaginary part (each with its axis-1 factor; Nyquist
-//    column packed into column 0, as k_ladder_rows); the first pass runs
-//    from registers and the last one leaves its outputs in registers, where
-//    the VMR ratio and Welford update of each point are formed.
+//    field in the imaginary part (Nyquist column packed into column 0, as
+//    k_ladder_rows, with its two axis-1 factors); the first pass runs from
+//    registers and the last one leaves its outputs in registers, where the
+//    VMR ratio and Welford update of each point are formed.
 // Against the axis-3 pass into tiled rows followed by k_ladder_rows, the
 // rung's fields are read once and not written again.  Eight waves keep every
 // lane of both transforms busy at 64 x 64 (64 pencils of 8 lanes, 8 rows of
@@ -2115,7 +2127,18 @@ template <int LOG2N1, int LOG2N3, int SPLIT>
 __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   constexpr int n1 = 1 << LOG2N1, H = n1 / 2, n3 = 1 << LOG2N3;
   constexpr int E = 8;
-  constexpr int PL1 = pencil_stride(n1), PL3 = pencil_stride(n3);
+  // LDS banks on this chip, measured: a wave's 16-byte reads are served 16
+  // lanes (256 bytes) at a time and its 16-byte writes 8 lanes at a time, and
+  // lanes of one group that hit the same 4-byte bank with different
+  // addresses conflict.  At 64 x 64 the planes sit 72 values apart (8 mod 16),
+  // so the two 8-lane pencils of a read group fill 16 different 16-byte
+  // slots, and the rows' lanes are numbered column-major (t1 = lane / 8), so
+  // that the gather of the row inputs from the planes reads 8 rows of two
+  // columns per group; with 73 and row-major lanes both had a 2-way conflict.
+  // Writes (a pencil's 8 lanes at stride 9, or 8 rows 73 apart) stay
+  // conflict-free.  64^3 ladder 45.0 -> 43.4 ms.
+  constexpr bool kBanks64 = LOG2N1 == 6 && LOG2N3 == 6 && SPLIT == 1;
+  constexpr int PL1 = pencil_stride(n1), PL3 = kBanks64 ? n3 + n3 / 8 : pencil_stride(n3);
   constexpr int NW = kSlabThreads / 64;
   constexpr int RW = n3 / NW;                  // rows (values of j3) per wave
   constexpr int lp1 = n1 / E, lp3 = n3 / E;    // lanes per row / per pencil
@@ -2163,10 +2186,6 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   const cplx<double>* src = a.src + (int64_t)rr * a.src_rs + (int64_t)j2 * (n3 * SPLIT) +
                             (int64_t)pl * B * a.fsize + (int64_t)k1p * a.Q + pcls * n3 + t3;
   cplx<double>* pb3 = P + (act3 ? pen : 0) * PL3;
-  double sf[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i)
-    sf[i] = SPLIT == 1 ? (pl ? gv : gm)[a.goff3 + t3 + i * nb3] : 1.0;  // split: applied on store
   cplx<double> nx[E], ny[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) nx[i] = ny[i] = {0.0, 0.0};
@@ -2178,26 +2197,20 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   };
   // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
   // input i (= t1 + i nb1) and its factors
-  const int w = lane / lp1, t1 = lane % lp1;
+  const int w = kBanks64 ? lane % RW : lane / lp1, t1 = kBanks64 ? lane / RW : lane % lp1;
   const bool act1 = w < RW;
   const int j3 = wv * RW + (act1 ? w : 0);
   cplx<double>* pb1 = Zall + (wv * RW + (act1 ? w : 0)) * PL1;
   int col[E];
-  double fm[E], fv[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int k = t1 + i * nb1;
     col[i] = k < H ? k : (k == H ? 0 : n1 - k);
-    fm[i] = gm[col[i]];
-    fv[i] = gv[col[i]];
   }
-  if (t1 == 0) {
-    // inputs 0 and E/2 are columns 0 and n1/2, packed in column 0
-    fm[0] = gm[0];
-    fv[0] = gv[0];
-    fm[E / 2] = gm[H];
-    fv[E / 2] = gv[H];
-  }
+  // the axis-2 pass applied the axis-3 and axis-1 factors, except the
+  // axis-1 factors of the packed column (k1 = 0 and n1/2: inputs 0 and E/2
+  // of the lanes with t1 = 0)
+  const double fm0 = gm[0], fv0 = gv[0], fmH = gm[H], fvH = gv[H];
   double mean[E], m2[E];
 #pragma unroll
   for (int u = 0; u < E; ++u) {
@@ -2220,7 +2233,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     const double inv = a.invk[b];
     cplx<double> v[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = cscale(cur[i], sf[i]);
+    for (int i = 0; i < E; ++i) v[i] = cur[i];
     load(cur, b + kSlabDepth < B ? b + kSlabDepth : b);
     __syncthreads();  // every wave is done reading the previous planes
     wave_first_pass<LOG2N3, E, 1, double>(pb3, t3, act3, v);
@@ -2231,9 +2244,8 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const cplx<double> pa = P[col[i] * PL3 + pad8(j3)];
-      const cplx<double> pq = P[(H + col[i]) * PL3 + pad8(j3)];
-      const cplx<double> A = cscale(pa, fm[i]), Bv = cscale(pq, fv[i]);
+      const cplx<double> A = P[col[i] * PL3 + pad8(j3)];
+      const cplx<double> Bv = P[(H + col[i]) * PL3 + pad8(j3)];
       if (i < E / 2)
         v[i] = {A.re - Bv.im, A.im + Bv.re};  // column k = t1 + i nb1 < n1/2
       else
@@ -2242,8 +2254,8 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
     if (t1 == 0) {
       // packed column: real part k1 = 0, imaginary part k1 = n1/2
       const cplx<double> pa = P[pad8(j3)], pq = P[H * PL3 + pad8(j3)];
-      v[0] = {pa.re * fm[0], pq.re * fv[0]};
-      v[E / 2] = {pa.im * fm[E / 2], pq.im * fv[E / 2]};
+      v[0] = {pa.re * fm0, pq.re * fv0};
+      v[E / 2] = {pa.im * fmH, pq.im * fvH};
     }
     wave_first_pass<LOG2N1, E, 1, double>(pb1, t1, act1, v);
     if constexpr (rt1)

@@ -112,7 +112,7 @@ template <typename T>
 bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                         int64_t dst_rung_stride, int nf, int R, int r0, const T* gM,
                         const T* gV, int fsplit, const T* tw, int mode, hipStream_t s,
-                        int split_target = kAxisSplitTarget);
+                        int split_target = kAxisSplitTarget, bool fold = false);
 
 // 3-D: inverse pass along grid axis 3 from the [k1][j2][j3] layout to the
 // tiled row-kernel layout.  False if n3 is outside the wave-local range.
