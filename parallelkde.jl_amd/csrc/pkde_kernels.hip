@@ -2138,7 +2138,17 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   // Writes (a pencil's 8 lanes at stride 9, or 8 rows 73 apart) stay
   // conflict-free.  64^3 ladder 45.0 -> 43.4 ms.
   constexpr bool kBanks64 = LOG2N1 == 6 && LOG2N3 == 6 && SPLIT == 1;
-  constexpr int PL1 = pencil_stride(n1), PL3 = kBanks64 ? n3 + n3 / 8 : pencil_stride(n3);
+  // The 128 x 32 quarter: pencils of 4 lanes 36 values apart (4 mod 16), rows
+  // 148 apart (4 mod 16), and each 16-lane group of the row phase takes 4 rows
+  // of 4 consecutive columns, so that the axis-3 passes (4 pencils per read
+  // group, 2 per write group), the gather (4 rows x 4 columns) and the row
+  // passes all fill distinct slots; with 37 / 145 and 16-lane rows the axis-3
+  // accesses and the row reads had 2-way conflicts.
+  constexpr bool kBanksQ = LOG2N1 == 7 && LOG2N3 == 5 && SPLIT == 4;
+  constexpr int PL1 = kBanksQ ? n1 + n1 / 8 + 4 : pencil_stride(n1);
+  constexpr int PL3 = kBanks64 || kBanksQ ? n3 + n3 / 8 : pencil_stride(n3);
+  static_assert(2 * H * PL3 + n3 * PL1 <= n1 * pencil_stride(n3) + n3 * pencil_stride(n1),
+                "the planes and rows fit the LDS that slab3_lds allocates");
   constexpr int NW = kSlabThreads / 64;
   constexpr int RW = n3 / NW;                  // rows (values of j3) per wave
   constexpr int lp1 = n1 / E, lp3 = n3 / E;    // lanes per row / per pencil
@@ -2197,7 +2207,8 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   };
   // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
   // input i (= t1 + i nb1) and its factors
-  const int w = kBanks64 ? lane % RW : lane / lp1, t1 = kBanks64 ? lane / RW : lane % lp1;
+  const int w = kBanks64 ? lane % RW : kBanksQ ? (lane >> 2) & 3 : lane / lp1;
+  const int t1 = kBanks64 ? lane / RW : kBanksQ ? ((lane >> 4) << 2) | (lane & 3) : lane % lp1;
   const bool act1 = w < RW;
   const int j3 = wv * RW + (act1 ? w : 0);
   cplx<double>* pb1 = Zall + (wv * RW + (act1 ? w : 0)) * PL1;
