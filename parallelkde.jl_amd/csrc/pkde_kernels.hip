@@ -2205,8 +2205,8 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
       for (int i = 0; i < E; ++i) dst[i] = chunk_load(src + (int64_t)bb * a.fsize + i * nb3);
     }
   };
-  // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column of
-  // input i (= t1 + i nb1) and its factors
+  // axis-1 row of this lane: j3 = wv RW + w, its thread t1; the column that
+  // input i (= t1 + i nb1) comes from
   const int w = kBanks64 ? lane % RW : kBanksQ ? (lane >> 2) & 3 : lane / lp1;
   const int t1 = kBanks64 ? lane / RW : kBanksQ ? ((lane >> 4) << 2) | (lane & 3) : lane % lp1;
   const bool act1 = w < RW;
