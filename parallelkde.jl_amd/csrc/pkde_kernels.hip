@@ -650,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // the n3/4-point inverse transform of c_p over k' gives the outputs
 // j3 = 4 m + p (k_slab3 with SPLIT = 4).  The arithmetic is that of
 // k_axis_wave with DIR = +1.
-template <typename T, int LOG2L, int NT = kThreads>
+template <typename T, int LOG2L, int MODE, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
   constexpr int L = 1 << LOG2L;
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
   bid /= (uint32_t)a.nsplit;
   const int rbeg = sp * a.R / a.nsplit, rend = (sp + 1) * a.R / a.nsplit;
   uint32_t kt, ot;
-  if (a.mode == 3) {
+  if constexpr (MODE == 3) {
     // outer tiles fastest: blocks that run together fill one k1 column
     ot = bid & ((1u << a.log2otiles) - 1u);
     bid >>= a.log2otiles;
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
   // the pencil's outer index: o0 + jj, or in mode 3 k3 = k' + s n3/4 with
   // k' = Cq ot + (jj mod Cq), s = jj / Cq, Cq = TJ / 4 values of k'
   const int log2Cq = log2TJ - 2, Cq = 1 << log2Cq;
-  const int64_t oq = a.mode == 3 ? (int64_t)ot * Cq + (jj & (Cq - 1)) +
+  const int64_t oq = MODE == 3 ? (int64_t)ot * Cq + (jj & (Cq - 1)) +
                                        (int64_t)(jj >> log2Cq) * (a.O >> 2)
                                  : o0 + jj;
   cplx<T> x[E];
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
   const int log2np = log2TK + log2TJ;
   const int total = 1 << (log2np + LOG2L);
   const int TKw = tile_cols_written<T>(TKe, TK);
-  if (a.mode == 1 && TKe < TKw) {
+  if (MODE == 1 && TKe < TKw) {
     // columns past H1 up to the end of the 128-byte line: zero pencils,
     // stored with the tile (no pass writes them: their lanes are inactive)
     for (int id = tid; id < ((TKw - TKe) << LOG2L); id += NT)
@@ -737,7 +737,8 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
     // fold: the pencil's axis-3 factor (k3 = oq) times its axis-1 factor
     // (k1 = k0 + kk; the axis-1 tables are at offset 0), one scalar per rung
     T g13 = (T)1;
-    if (a.fold) {
+    const bool fold = MODE == 3 || (MODE == 2 && a.fold);
+    if (fold) {
       const T* gr = gt - a.goff;
       const int kc = act ? k0 + kk : 0;
       const T g3 = gr[a.goff3 + (act ? oq : 0)];
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
         cplx<T> w[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r)
-          w[r] = cscale(x[u * R1 + r], a.fold ? gt[j + r * nb1] * g13 : gt[j + r * nb1]);
+          w[r] = cscale(x[u * R1 + r], fold ? gt[j + r * nb1] * g13 : gt[j + r * nb1]);
         dft_reg<R1, 1, T>(w);
 #pragma unroll
         for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
     wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
-    if (a.mode == 1) {
+    if constexpr (MODE == 1) {
       // tiled rows (2-D, TJ == 1, TK == TKy): the block's tile is contiguous;
       // rows are written in whole 128-byte lines (zero columns pad the last
       // tile), since stores that cover only part of a line cost far more
@@ -771,7 +772,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
         const int e = id >> log2TK;
         if (k < TKw) chunk_store(tile + (o0 * L + e) * TK + k, buf[k * PL + pad8(e)]);
       }
-    } else if (a.mode == 2) {
+    } else if constexpr (MODE == 2) {
       // [k1][j2][j3]: consecutive j3 (outer index) for one (k1, j2)
       for (int id = tid; id < total; id += NT) {
         const int j = id & (TJ - 1);
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
           chunk_store(dst + (int64_t)(k0 + k) * a.Q + (int64_t)e * a.O + o0 + j,
                       buf[((k << log2TJ) + j) * PL + pad8(e)]);
       }
-    } else if (a.mode == 3) {
+    } else if constexpr (MODE == 3) {
       // class-major radix-4 step: a thread takes one (k1, k' = Cq ot + c, j2),
       // reads the four values Y(k' + s n3/4) once and stores all four classes
       // at [p][k']; 8 lanes write the 8 values of k' of one class, so every
@@ -833,18 +834,54 @@ template <int LOG2L>
 static bool axis_ladder512(unsigned blocks, size_t lds, hipStream_t s,
                            const AxisArgs<double>& a) {
   static const bool attr = [] {
-    (void)hipFuncSetAttribute((const void*)k_axis_ladder<double, LOG2L, 512>,
+    (void)hipFuncSetAttribute((const void*)k_axis_ladder<double, LOG2L, 3, 512>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
     return true;
   }();
   (void)attr;
-  k_axis_ladder<double, LOG2L, 512><<<blocks, 512, lds, s>>>(a);
+  k_axis_ladder<double, LOG2L, 3, 512><<<blocks, 512, lds, s>>>(a);
   return true;
+}
+
+// The store layout is a template parameter, so each kernel carries only its
+// own store (with the layout chosen at run time the 2-D pass ran 1.6 % slower).
+template <typename T, int MODE>
+static bool axis_ladder_launch_mode(int log2L, unsigned blocks, size_t lds, hipStream_t s,
+                                    const AxisArgs<T>& a) {
+  if constexpr (sizeof(T) == 8) {
+    switch (log2L) {
+      case 3: k_axis_ladder<T, 3, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 4: k_axis_ladder<T, 4, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 5: k_axis_ladder<T, 5, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 6: k_axis_ladder<T, 6, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 7: k_axis_ladder<T, 7, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 8: k_axis_ladder<T, 8, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 9: k_axis_ladder<T, 9, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      default: return false;
+    }
+  } else {
+    switch (log2L) {
+      case 4: k_axis_ladder<T, 4, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 5: k_axis_ladder<T, 5, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 6: k_axis_ladder<T, 6, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 7: k_axis_ladder<T, 7, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 8: k_axis_ladder<T, 8, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 9: k_axis_ladder<T, 9, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      case 10: k_axis_ladder<T, 10, MODE><<<blocks, kThreads, lds, s>>>(a); return true;
+      default: return false;
+    }
+  }
 }
 
 template <typename T>
 static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream_t s,
                                const AxisArgs<T>& a) {
+  switch (a.mode) {
+    case 0: return axis_ladder_launch_mode<T, 0>(log2L, blocks, lds, s, a);
+    case 1: return axis_ladder_launch_mode<T, 1>(log2L, blocks, lds, s, a);
+    case 2: return axis_ladder_launch_mode<T, 2>(log2L, blocks, lds, s, a);
+    default: break;
+  }
   if constexpr (sizeof(T) == 8) {
     if (a.mode == 3) {
       // 32 pencils per block, 512 threads, 72 KB of LDS (above the 64 KB
@@ -858,28 +895,8 @@ static bool axis_ladder_launch(int log2L, unsigned blocks, size_t lds, hipStream
         default: return false;
       }
     }
-    switch (log2L) {
-      case 3: k_axis_ladder<T, 3><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 4: k_axis_ladder<T, 4><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 5: k_axis_ladder<T, 5><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 6: k_axis_ladder<T, 6><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 7: k_axis_ladder<T, 7><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 8: k_axis_ladder<T, 8><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 9: k_axis_ladder<T, 9><<<blocks, kThreads, lds, s>>>(a); return true;
-      default: return false;
-    }
-  } else {
-    switch (log2L) {
-      case 4: k_axis_ladder<T, 4><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 5: k_axis_ladder<T, 5><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 6: k_axis_ladder<T, 6><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 7: k_axis_ladder<T, 7><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 8: k_axis_ladder<T, 8><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 9: k_axis_ladder<T, 9><<<blocks, kThreads, lds, s>>>(a); return true;
-      case 10: k_axis_ladder<T, 10><<<blocks, kThreads, lds, s>>>(a); return true;
-      default: return false;
-    }
   }
+  return false;
 }
 
 // First inverse pass of a rung chunk along grid axis 2 for the 2B bootstrap
