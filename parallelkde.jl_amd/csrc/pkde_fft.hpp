@@ -390,14 +390,28 @@ __host__ __device__ constexpr int pencil_stride(int L) { return L + L / 8 + 1; }
 
 __host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
+// Slot of element p of a wave-local pencil of E values per lane: SW = 0 the
+// padded layout (pad8); SW = 1 an unpadded XOR swizzle, p ^ ((p / E) mod 8).
+// LDS serves a wave's 16-byte reads 16 lanes at a time and its writes 8 at a
+// time, and lanes of one group on the same bank conflict.  The padded layout
+// keeps stride-8 writes conflict-free but puts one 2-way conflict in every
+// read of 16 consecutive values; the swizzle permutes only the low 3 bits
+// within each run of 8, so reads of 16 aligned consecutive values and the
+// first pass's stride-E writes both fill distinct banks.
+template <int SW, int LE>
+__host__ __device__ constexpr int lds_slot(int p) {
+  return SW ? (p ^ ((p >> LE) & 7)) : pad8(p);
+}
+
 // One Stockham pass of radix R over the lane's own pencil `pb` (length
 // L = 2^LOG2L, padded layout), the lane being thread t of the pencil's L/E
 // threads; Ns = 2^LOG2NS is the length of the sub-transforms already done.
 // All sizes are compile-time, so index arithmetic reduces to shifts and masks.
 // All lanes of the wave call it (active = false for lanes without a pencil).
-template <int R, int E, int LOG2L, int LOG2NS, int DIR, typename T>
+template <int R, int E, int LOG2L, int LOG2NS, int DIR, typename T, int SW = 0>
 __device__ __forceinline__ void wave_pass(cplx<T>* pb, int t, bool active,
                                           const cplx<T>* __restrict__ tw) {
+  constexpr int LE = ilog2c(E);
   constexpr int L = 1 << LOG2L;
   constexpr int tpp = L / E;
   constexpr int nbfly = L / R;
@@ -410,7 +424,7 @@ __device__ __forceinline__ void wave_pass(cplx<T>* pb, int t, bool active,
     for (int u = 0; u < E / R; ++u) {
       const int j = t + u * tpp;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[u * R + r] = pb[pad8(j + r * nbfly)];
+      for (int r = 0; r < R; ++r) v[u * R + r] = pb[lds_slot<SW, LE>(j + r * nbfly)];
     }
   }
   wave_sync();
@@ -431,22 +445,22 @@ __device__ __forceinline__ void wave_pass(cplx<T>* pb, int t, bool active,
       dft_reg<R, DIR, T>(w);
       const int out0 = ((j >> LOG2NS) << (LOG2NS + LOG2R)) + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) pb[pad8(out0 + r * Ns)] = w[r];
+      for (int r = 0; r < R; ++r) pb[lds_slot<SW, LE>(out0 + r * Ns)] = w[r];
     }
   }
   wave_sync();
 }
 
-template <int LOG2L, int LOG2NS, int E, int DIR, typename T>
+template <int LOG2L, int LOG2NS, int E, int DIR, typename T, int SW = 0>
 __device__ __forceinline__ void wave_fft_from(cplx<T>* pb, int t, bool active,
                                               const cplx<T>* __restrict__ tw) {
   constexpr int LE = ilog2c(E);
   constexpr int rem = LOG2L - LOG2NS;
   if constexpr (rem >= LE) {
-    wave_pass<E, E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw);
-    wave_fft_from<LOG2L, LOG2NS + LE, E, DIR, T>(pb, t, active, tw);
+    wave_pass<E, E, LOG2L, LOG2NS, DIR, T, SW>(pb, t, active, tw);
+    wave_fft_from<LOG2L, LOG2NS + LE, E, DIR, T, SW>(pb, t, active, tw);
   } else if constexpr (rem > 0) {
-    wave_pass<(1 << rem), E, LOG2L, LOG2NS, DIR, T>(pb, t, active, tw);
+    wave_pass<(1 << rem), E, LOG2L, LOG2NS, DIR, T, SW>(pb, t, active, tw);
   }
 }
 
@@ -517,11 +531,11 @@ __device__ __forceinline__ void wave_fft_to_regs(cplx<T>* pb, int t, bool active
 // E = 16 gives radix-16 passes; E = 8 (radix-8 passes) halves the registers
 // per lane, which the FP64 kernels need for occupancy.  Natural order in and
 // out; `tw` is the table exp(-2 pi i k / L), k < L.
-template <int LOG2L, int DIR, typename T, int E>
+template <int LOG2L, int DIR, typename T, int E, int SW = 0>
 __device__ __forceinline__ void wave_fft(cplx<T>* pb, int t, bool active,
                                          const cplx<T>* __restrict__ tw) {
   static_assert((1 << LOG2L) >= E && (1 << LOG2L) <= 64 * E, "pencil must fit one wavefront");
-  wave_fft_from<LOG2L, 0, E, DIR, T>(pb, t, active, tw);
+  wave_fft_from<LOG2L, 0, E, DIR, T, SW>(pb, t, active, tw);
 }
 
 // ------------------------------- wave-local passes, twiddles in registers

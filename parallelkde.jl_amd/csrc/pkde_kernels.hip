@@ -1485,30 +1485,34 @@ __global__ __launch_bounds__(64 * kWaves1d) void k_ladder_1d(VmrArgs<T> a) {
   }
 }
 
-// 1-D ladder for 1024-point grids in FP64, one block of kWavesMs wavefronts
-// per rung.  A 1024-point pencil pair per wave does not fit LDS (2 pencils
-// of 18 KB per wave), so each wavefront takes ONE bootstrap b per round and
+// 1-D ladder for 512- and 1024-point grids in FP64, one block of NW
+// wavefronts per rung.  Each wavefront takes ONE bootstrap b per round and
 // transforms z = m + i s, the sums field m and the squares field s of that
 // bootstrap (both real), as one complex pencil: the real part of the result
 // is the sums field and the imaginary part the squares field, the two terms
-// of b's VMR ratio.  The transform has 16 values per lane (radix-16
-// passes); 7 wavefronts leave room in LDS for the twiddle table.  After a
-// barrier every thread runs the Welford updates of its points over the
-// round's bootstraps in order.  Same arithmetic as k_ladder_1d otherwise.
-constexpr int kWavesMs = 7;
-
+// of b's VMR ratio.  At 1024 points a pencil pair per wave does not fit LDS
+// (2 pencils of 18 KB per wave): 7 wavefronts with 16 values per lane
+// (radix-16 passes) leave room for the twiddle table.  At 512 points the
+// single pencils let 16 wavefronts share the CU where k_ladder_1d's pairs
+// fit 8 (radix-8 passes).  After a barrier every thread runs the Welford
+// updates of its points over the round's bootstraps in order.  Same
+// arithmetic as k_ladder_1d otherwise.
 template <int LOG2N1>
-__global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> a) {
+constexpr int ladder_1d_ms_waves() { return LOG2N1 == 10 ? 8 : 16; }
+
+template <int LOG2N1, int NW>
+__global__ __launch_bounds__(64 * NW) void k_ladder_1d_ms(VmrArgs<double> a) {
   constexpr int n1 = 1 << LOG2N1;
   constexpr int H = n1 / 2;
   constexpr int E = n1 / 64;  // one pencil per wavefront
-  constexpr int PL = pencil_stride(n1);
-  constexpr int NT = 64 * kWavesMs;
+  constexpr int PL = n1;  // unpadded pencils, XOR-swizzled slots (lds_slot<1>)
+  constexpr int LE = ilog2c(E);
+  constexpr int NT = 64 * NW;
   constexpr int PPT = (n1 + NT - 1) / NT;  // points per thread in the Welford phase
-  static_assert(E == 16, "1024-point pencils, 16 values per lane");
+  static_assert(E == 8 || E == 16, "512- or 1024-point pencils, 8 or 16 values per lane");
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  cplx<double>* twl = (cplx<double>*)smem + kWavesMs * PL;
+  cplx<double>* twl = (cplx<double>*)smem + NW * PL;
   for (int k = tid; k < n1; k += NT) twl[k] = a.tw[n1 - 2 + k];
   __syncthreads();
   cplx<double>* Z = (cplx<double>*)smem + wv * PL;
@@ -1542,7 +1546,7 @@ __global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> 
     }
   };
   issue(wv);
-  for (int b0 = 0; b0 < B; b0 += kWavesMs) {
+  for (int b0 = 0; b0 < B; b0 += NW) {
     const int b = b0 + wv;
     if (b < B) {
       // Z[k] = M[k] + i S[k], Z[n1 - k] = conj(M[k]) + i conj(S[k]), with M
@@ -1553,8 +1557,8 @@ __global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> 
         const int id = lane + 64 * i;
         if (id <= H) {
           const cplx<double> M = cscale(la[i], gm1[id]);
-          Z[pad8(id)] = M;
-          if (id > 0 && id < H) Z[pad8(n1 - id)] = cconj(M);
+          Z[lds_slot<1, LE>(id)] = M;
+          if (id > 0 && id < H) Z[lds_slot<1, LE>(n1 - id)] = cconj(M);
         }
       }
       wave_sync();
@@ -1564,34 +1568,34 @@ __global__ __launch_bounds__(64 * kWavesMs) void k_ladder_1d_ms(VmrArgs<double> 
         if (id > H && id < 2 * (H + 1)) {
           const int k1 = id - H - 1;
           const cplx<double> S = cscale(la[i], gv1[k1]);
-          const cplx<double> m = Z[pad8(k1)];
-          Z[pad8(k1)] = {m.re - S.im, m.im + S.re};
+          const cplx<double> m = Z[lds_slot<1, LE>(k1)];
+          Z[lds_slot<1, LE>(k1)] = {m.re - S.im, m.im + S.re};
           if (k1 > 0 && k1 < H) {
-            const cplx<double> mc = Z[pad8(n1 - k1)];
-            Z[pad8(n1 - k1)] = {mc.re + S.im, mc.im + S.re};
+            const cplx<double> mc = Z[lds_slot<1, LE>(n1 - k1)];
+            Z[lds_slot<1, LE>(n1 - k1)] = {mc.re + S.im, mc.im + S.re};
           }
         }
       }
       wave_sync();
-      wave_fft<LOG2N1, 1, double, E>(Z, lane, true, twl);
+      wave_fft<LOG2N1, 1, double, E, 1>(Z, lane, true, twl);
       // the ratio of bootstrap b replaces the pencil (as doubles in the
       // first 8 bytes of each slot; every lane rewrites what it read)
       for (int j = lane; j < n1; j += 64) {
-        const cplx<double> z = Z[pad8(j)];
-        *(double*)(Z + pad8(j)) = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
+        const cplx<double> z = Z[lds_slot<1, LE>(j)];
+        *(double*)(Z + lds_slot<1, LE>(j)) = vmr_ratio(fabs(z.im) * vfac, fabs(z.re) * a.invG, a.invn);
       }
     }
-    issue(b + kWavesMs);
+    issue(b + NW);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int j = tid + u * NT;
       if (j < n1) {
-        for (int w = 0; w < kWavesMs; ++w) {
+        for (int w = 0; w < NW; ++w) {
           const int bw = b0 + w;
           if (bw >= B) break;
           const cplx<double>* Zw = (const cplx<double>*)smem + w * PL;
-          const double xv = *(const double*)(Zw + pad8(j));
+          const double xv = *(const double*)(Zw + lds_slot<1, LE>(j));
           const double d = xv - mean[u];
           mean[u] += d / (double)(bw + 1);
           m2[u] += d * (xv - mean[u]);
@@ -1755,6 +1759,21 @@ static void ladder_1d_launch(unsigned blocks, hipStream_t s, const VmrArgs<T>& a
 
 // True if the 1-D ladder took the wave-parallel form (n1 a power of two in
 // the wave-local FFT range, at least 16).
+template <int LOG2N1>
+static void ladder_1d_ms_launch(unsigned blocks, hipStream_t s, const VmrArgs<double>& a) {
+  constexpr int n1 = 1 << LOG2N1, NW = ladder_1d_ms_waves<LOG2N1>();
+  constexpr size_t lds = ((size_t)NW * n1 + n1) * sizeof(cplx<double>);  // unpadded pencils
+  static_assert(lds <= 160 * 1024, "the wavefronts' pencils fit the LDS of a CU");
+  static const bool attr = [] {
+    // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
+    (void)hipFuncSetAttribute((const void*)k_ladder_1d_ms<LOG2N1, NW>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return true;
+  }();
+  (void)attr;
+  k_ladder_1d_ms<LOG2N1, NW><<<blocks, 64 * NW, lds, s>>>(a);
+}
+
 template <typename T>
 static bool ladder_1d_dispatch(int log2n1, unsigned blocks, hipStream_t s, const VmrArgs<T>& a) {
   switch (log2n1) {
@@ -1763,22 +1782,13 @@ static bool ladder_1d_dispatch(int log2n1, unsigned blocks, hipStream_t s, const
     case 6: ladder_1d_launch<T, 6>(blocks, s, a); return true;
     case 7: ladder_1d_launch<T, 7>(blocks, s, a); return true;
     case 8: ladder_1d_launch<T, 8>(blocks, s, a); return true;
-    case 9: ladder_1d_launch<T, 9>(blocks, s, a); return true;
+    case 9:
+      if constexpr (sizeof(T) == 4) ladder_1d_launch<T, 9>(blocks, s, a);
+      else ladder_1d_ms_launch<9>(blocks, s, a);
+      return true;
     case 10:
-      if constexpr (sizeof(T) == 4) {
-        ladder_1d_launch<T, 10>(blocks, s, a);
-      } else {
-        constexpr size_t lds =
-            ((size_t)kWavesMs * pencil_stride(1024) + 1024) * sizeof(cplx<double>);
-        static const bool attr = [] {
-          // beyond 64 KB of LDS per block (gfx950 has 160 KB per CU); set once
-          (void)hipFuncSetAttribute((const void*)k_ladder_1d_ms<10>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-          return true;
-        }();
-        (void)attr;
-        k_ladder_1d_ms<10><<<blocks, 64 * kWavesMs, lds, s>>>(a);
-      }
+      if constexpr (sizeof(T) == 4) ladder_1d_launch<T, 10>(blocks, s, a);
+      else ladder_1d_ms_launch<10>(blocks, s, a);
       return true;
     default: return false;
   }
