@@ -636,13 +636,15 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // (R / nsplit), so the spectra are read once per share rather than once per
 // rung.  Per rung:
 // Gaussian factor, first pass from registers, remaining passes through LDS,
-// store of the block's pencils in layout `mode`: 0 [k1][q], 1 the tiled rows
+// store of the block's pencils in layout MODE: 0 [k1][q], 1 the tiled rows
 // of the row kernel (2-D; the block's tile is one contiguous run), 2
-// [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass), 3 the first
+// [k1][j2][j3] (axis 3 contiguous, for the 3-D axis-3 pass or k_slab3; for
+// k_slab3 (a.fold) the pencil's axis-3 and axis-1 factors go in with the
+// axis-2 factors, except the axis-1 factors of the packed column 0), 3 the first
 // radix-4 decimation-in-frequency step of the axis-3 inverse transform applied
 // on store (NT = 512 threads): the block's 32 pencils are k3 = k' + s n3/4
-// for 8 consecutive k' and s = 0..3 (each scaled by its axis-3 factor g3(k3)
-// with the axis-2 factors), and it stores, for each j2 and residue class
+// for 8 consecutive k' and s = 0..3 (each scaled by its axis-3 and axis-1
+// factors with the axis-2 factors), and it stores, for each j2 and residue class
 // p = j3 mod 4, c_p(k') = w^(p k') sum_s i^(p s) Y(k' + s n3/4), w =
 // exp(2 pi i / n3), class-major at [k1][j2][p][k']: each class's values of a
 // (k1, j2) are one contiguous run, written by the blocks in whole 128-byte
