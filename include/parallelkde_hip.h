@@ -227,6 +227,19 @@ int pkde_dirac_sequences(pkde_ctx *ctx, int ndims, const int64_t *n_per_axis,
                          const double *samples, int64_t n, const int32_t *bootstrap_idx,
                          int32_t B, int deterministic, double *sums_out, double *squares_out);
 
+/* Forward transform alone (plan_fft! + fourier_statistics!,
+ * GradeProEstimator.jl:90-95, FourierSpace.jl:25-80), for primitive-level
+ * parity: the unnormalised forward DFT (exp(-2 pi i j k / n) along every
+ * axis) of `count` real fields on the plan's grid, with the kernels the
+ * estimate runs (two fields per complex axis-1 transform, then the passes
+ * along the other axes).  fields: count x G doubles, grid axis 1 fastest;
+ * spectra_out: count x (n1/2 + 1) x (G / n1) complex values as interleaved
+ * (re, im) doubles, [field][k1][q] with q = k2 + n2 k3 + ... (the Hermitian
+ * half: X(n1 - k1, -q) = conj X(k1, q)).  1 <= count <= 2 B + 1 of the plan.
+ * Uses the plan's binning and spectrum buffers (the next estimate overwrites
+ * them). */
+int pkde_fft_forward(pkde_plan *plan, int32_t count, const double *fields, double *spectra_out);
+
 /* ---- batched form: many independent estimates in one submission ----
  * Requests are spread over up to PKDE_BATCH_STREAMS (default 8) device
  * streams and run concurrently.  Each stream keeps one plan per grid shape

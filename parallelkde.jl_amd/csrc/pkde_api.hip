@@ -1435,6 +1435,38 @@ int pkde_dirac_sequences(pkde_ctx* ctx, int ndims, const int64_t* n_per_axis, co
   return PKDE_OK;
 }
 
+// plan_fft! + fourier_statistics! (GradeProEstimator.jl:90-95,
+// FourierSpace.jl:25-80) alone: the estimate's forward kernels on given real
+// fields (FP64 accumulator mode, scale 1), for primitive-level parity.
+int pkde_fft_forward(pkde_plan* pl, int32_t count, const double* fields, double* spectra_out) {
+  if (!pl || !fields || !spectra_out) return fail(PKDE_INVALID_ARGUMENT, "null argument");
+  if (!pl->ctx) return fail(PKDE_INVALID_ARGUMENT, "the plan's context was destroyed");
+  const int64_t F = 2 * (int64_t)pl->B + 1;
+  if (count < 1 || count > F) return fail(PKDE_INVALID_ARGUMENT, "count must be in 1 .. 2 B + 1");
+  std::lock_guard<std::mutex> lk(pl->ctx->mu);
+  PKDE_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  hipStream_t s = pl->ctx->stream;
+  (void)hipGetLastError();
+  int rc = plan_reserve(pl, 2, false);
+  if (rc) return rc;
+  PlanBuffers& d = pl->d;
+  const GridDev& g = pl->g;
+  const int64_t G = g.G, fs = fsize(pl);
+  PKDE_TRY(hipMemcpyAsync(d.acc, fields, (size_t)count * G * 8, hipMemcpyHostToDevice, s),
+           "upload fields");
+  if (count < F) launch_zero_bytes((unsigned char*)d.acc + (size_t)count * G * 8, (F - count) * G * 8, s);
+  launch_forward_axis1(g, d.acc, 0, 1.0, 1.0, pl->B, d.specD, pl->ctx->tw64, d.twg64, s);
+  for (int ax = 1; ax < g.N; ++ax)
+    launch_axis_fft<double, double, -1>(g, ax, d.specD, 0, 0, d.specD, 0, 0, 0, (int)F, 1, 0,
+                                        nullptr, nullptr, 0, pl->ctx->tw64, d.twg64, s);
+  PKDE_TRY(hipGetLastError(), "kernel launch");
+  PKDE_TRY(hipMemcpyAsync(spectra_out, d.specD, (size_t)count * fs * sizeof(cplx<double>),
+                          hipMemcpyDeviceToHost, s),
+           "download spectra");
+  PKDE_TRY(hipStreamSynchronize(s), "forward transform");
+  return PKDE_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------- allocation-free estimate

@@ -470,6 +470,17 @@ class _Plan:
                                                   _lib.C.byref(default)), "autotune")
         return best.value, default.value
 
+    def fft_forward(self, fields):
+        """Forward DFT of real fields with the estimate's own kernels
+        (pkde_fft_forward): fields (count, G) with grid axis 1 fastest ->
+        complex (count, n1 // 2 + 1, G // n1), [field][k1][q]."""
+        f = np.ascontiguousarray(np.asarray(fields, dtype=np.float64).reshape(len(fields), -1))
+        n1 = self.grid.shape[0]
+        out = np.empty((f.shape[0], n1 // 2 + 1, f.shape[1] // n1), dtype=np.complex128)
+        _raise(_lib.load().pkde_fft_forward(self.handle, f.shape[0], f.ctypes.data,
+                                            out.ctypes.data), "forward transform")
+        return out
+
     def debug_vmr(self, n_rungs):
         G = int(np.prod(self.grid.shape))
         out = np.empty(n_rungs * G)

@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = [
     "pkde_gradepro_last_stats",
     "pkde_gradepro_debug_enable", "pkde_gradepro_debug_vmr", "pkde_gradepro_convergence",
     "pkde_gradepro_graph_enable", "pkde_gradepro_autotune",
-    "pkde_dirac_sequences", "pkde_gradepro_estimate_batch", "pkde_context_release_batch",
+    "pkde_dirac_sequences", "pkde_fft_forward", "pkde_gradepro_estimate_batch",
+    "pkde_context_release_batch",
     "pkde_gradepro_estimate_batch_device", "pkde_gradepro_estimate_1d_noalloc",
     "pkde_gradepro_request_footprint",
     "pkde_server_create", "pkde_server_submit", "pkde_server_poll", "pkde_server_wait",
@@ -134,6 +135,7 @@ def load() -> C.CDLL:
                                             C.c_int32, vp]
     L.pkde_dirac_sequences.argtypes = [vp, C.c_int, i64p, dp, dp, dp, vp, C.c_int64, vp, C.c_int32,
                                        C.c_int, vp, vp]
+    L.pkde_fft_forward.argtypes = [vp, C.c_int32, vp, vp]
     L.pkde_gradepro_estimate_batch.argtypes = [vp, C.c_int32, C.POINTER(Request), vp]
     L.pkde_context_release_batch.argtypes = [vp]
     L.pkde_gradepro_estimate_1d_noalloc.argtypes = [vp, C.c_int64, C.c_double, C.c_double,
