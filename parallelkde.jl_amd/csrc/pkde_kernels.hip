@@ -1891,6 +1891,12 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
   constexpr int PL = pencil_stride(n1);
   constexpr int lp = n1 / E;               // lanes per pencil
   constexpr int ppc = 64 / lp;             // pencils per wave-wide FFT call
+  // FP64 pencils use the XOR-swizzled slots (lds_slot<1>): the transform's
+  // 16-lane reads of values 32 or 64 apart and the Welford reads of 16
+  // consecutive values fill distinct banks, where the padded layout has a
+  // 2-way conflict in each of them
+  constexpr int SWR = sizeof(T) == 8 ? 1 : 0;
+  constexpr int LE = ilog2c(E);
   static_assert(W >= 1 && (W & (W - 1)) == 0, "rows per wave must be a power of two");
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
@@ -1973,13 +1979,13 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
         // real-valued per field); A + i B for the pair at each
         const T gN = (rw >> LOG2W) ? gNv : gNm;
         const T br = has2 ? pb2[u].re : (T)0, bi = has2 ? pb2[u].im : (T)0;
-        row[pad8(0)] = {pa[u].re * ld_g[u], br * ld_g[u]};
-        row[pad8(H)] = {pa[u].im * gN, bi * gN};
+        row[lds_slot<SWR, LE>(0)] = {pa[u].re * ld_g[u], br * ld_g[u]};
+        row[lds_slot<SWR, LE>(H)] = {pa[u].im * gN, bi * gN};
       } else {
         const cplx<T> A = cscale(pa[u], ld_g[u]);
         const cplx<T> Bv = has2 ? cscale(pb2[u], ld_g[u]) : zero;
-        row[pad8(k1)] = {A.re - Bv.im, A.im + Bv.re};
-        if (k1 > 0) row[pad8(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
+        row[lds_slot<SWR, LE>(k1)] = {A.re - Bv.im, A.im + Bv.re};
+        if (k1 > 0) row[lds_slot<SWR, LE>(n1 - k1)] = {A.re + Bv.im, Bv.re - A.im};
       }
     }
     const double inv1 = pi1;
@@ -1989,7 +1995,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     for (int c = 0; c < 2 * W; c += ppc) {
       const int p = c + lane / lp;
       const bool act = p < 2 * W;
-      wave_fft<LOG2N1, 1, T, E>(Z + (act ? p : 0) * PL, lane % lp, act, twl);
+      wave_fft<LOG2N1, 1, T, E, SWR>(Z + (act ? p : 0) * PL, lane % lp, act, twl);
     }
     // next pair's loads (the last iteration reloads its own pair: valid
     // addresses, values unused)
@@ -1998,8 +2004,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_rows(VmrArgs<T> a) {
     for (int u = 0; u < PPT; ++u) {
       const int pt = lane + 64 * u;
       const int w = pt >> LOG2N1, j1 = pt & (n1 - 1);
-      const cplx<T> zm = Z[w * PL + pad8(j1)];
-      const cplx<T> zv = Z[(W + w) * PL + pad8(j1)];
+      const cplx<T> zm = Z[w * PL + lds_slot<SWR, LE>(j1)];
+      const cplx<T> zv = Z[(W + w) * PL + lds_slot<SWR, LE>(j1)];
       {
         const double am = fabs((double)zm.re) * a.invG;
         const double av = fabs((double)zv.re) * vfac;
