@@ -652,11 +652,30 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // the n3/4-point inverse transform of c_p over k' gives the outputs
 // j3 = 4 m + p (k_slab3 with SPLIT = 4).  The arithmetic is that of
 // k_axis_wave with DIR = +1.
+// Pencil stride of the ladder axis pass: the padded layout, or in mode 1
+// (FP64, L >= 256) unpadded pencils with the XOR swizzle (lds_slot<1>), L + 2
+// values apart.  There a pencil's 32 or more lanes read 16 values 32 apart
+// per group in the later passes, which the padded layout serves with a 2-way
+// bank conflict in every read; swizzled they fill distinct banks, and with
+// the pencils 2 (mod 16) values apart the tile store's reads (8 columns at
+// two neighbouring rows per 16-lane group) do too.  256^2: ladder 10.79 ->
+// 10.72 ms.  At L = 128 the slot arithmetic costs the pass a wave per SIMD
+// (126 -> 154 VGPRs) for no measured gain, so shorter axes stay padded.
+template <typename T>
+constexpr int axis_ladder_swizzled(int mode, int L) {
+  return mode == 1 && sizeof(T) == 8 && L >= 256 ? 1 : 0;
+}
+template <typename T>
+constexpr int axis_ladder_stride(int mode, int L) {
+  return axis_ladder_swizzled<T>(mode, L) ? L + 2 : pencil_stride(L);
+}
+
 template <typename T, int LOG2L, int MODE, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
   constexpr int L = 1 << LOG2L;
-  constexpr int PL = pencil_stride(L);
+  constexpr int SWZ = axis_ladder_swizzled<T>(MODE, L);
+  constexpr int PL = axis_ladder_stride<T>(MODE, L);
   constexpr int lp = L / E;                  // lanes per pencil
   constexpr int ppw = 64 / lp;               // pencils per wave
   constexpr int LE = ilog2c(E);
@@ -731,7 +750,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
     // columns past H1 up to the end of the 128-byte line: zero pencils,
     // stored with the tile (no pass writes them: their lanes are inactive)
     for (int id = tid; id < ((TKw - TKe) << LOG2L); id += NT)
-      buf[(TKe + (id >> LOG2L)) * PL + pad8(id & (L - 1))] = {(T)0, (T)0};
+      buf[(TKe + (id >> LOG2L)) * PL + lds_slot<SWZ, LE>(id & (L - 1))] = {(T)0, (T)0};
   }
 
   for (int rr = rbeg; rr < rend; ++rr) {
@@ -756,11 +775,11 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
           w[r] = cscale(x[u * R1 + r], fold ? gt[j + r * nb1] * g13 : gt[j + r * nb1]);
         dft_reg<R1, 1, T>(w);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) pb[pad8((j << LOG2R1) + r)] = w[r];
+        for (int r = 0; r < R1; ++r) pb[lds_slot<SWZ, LE>((j << LOG2R1) + r)] = w[r];
       }
     }
     wave_sync();
-    wave_fft_from<LOG2L, LOG2R1, E, 1, T>(pb, t, act, a.tw + (L - 2));
+    wave_fft_from<LOG2L, LOG2R1, E, 1, T, SWZ>(pb, t, act, a.tw + (L - 2));
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if constexpr (MODE == 1) {
@@ -772,7 +791,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
       for (int id = tid; id < total; id += NT) {
         const int k = id & (TK - 1);
         const int e = id >> log2TK;
-        if (k < TKw) chunk_store(tile + (o0 * L + e) * TK + k, buf[k * PL + pad8(e)]);
+        if (k < TKw) chunk_store(tile + (o0 * L + e) * TK + k, buf[k * PL + lds_slot<SWZ, LE>(e)]);
       }
     } else if constexpr (MODE == 2) {
       // [k1][j2][j3]: consecutive j3 (outer index) for one (k1, j2)
@@ -982,7 +1001,7 @@ bool launch_axis_ladder(const GridDev& g, const cplx<T>* spec, cplx<T>* dst,
                                                                (split_target + tiles - 1) / tiles,
                                                                R / 8));
   const int64_t blocks = tiles * a.nsplit;
-  const size_t lds = (size_t)a.TK * a.TJ * pencil_stride(a.L) * sizeof(cplx<T>);
+  const size_t lds = (size_t)a.TK * a.TJ * axis_ladder_stride<T>(mode, a.L) * sizeof(cplx<T>);
   return axis_ladder_launch<T>(a.log2L, (unsigned)blocks, lds, s, a);
 }
 
