@@ -260,6 +260,10 @@ def _check_fp64(m, envelope):
 CASES = [
     ("1d-512", ([-5.0], [5.0], [512]), lambda: W.gaussian_1d(10000, 1), 100, 250),
     ("2d-64", ([-6.0, -6.0], [6.0, 6.0], [64, 64]), lambda: W.mixture_2d(20000, 2), 40, 120),
+    # axis-2 lengths of 256 and 512: the ladder's axis pass in its swizzled
+    # LDS layout, with narrow rows (8 and 16 carried columns per field)
+    ("2d-16x256", ([-6.0, -6.0], [6.0, 6.0], [16, 256]), lambda: W.mixture_2d(20000, 60), 30, 100),
+    ("2d-32x512", ([-6.0, -6.0], [6.0, 6.0], [32, 512]), lambda: W.mixture_2d(20000, 61), 30, 100),
     # 1024 points: one bootstrap's sums and squares fields in one pencil
     ("1d-1024", ([-6.0], [6.0], [1024]), lambda: W.gaussian_1d(20000, 41, lo=-6.0, hi=6.0), 40,
      120),
