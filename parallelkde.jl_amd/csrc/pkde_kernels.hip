@@ -2284,7 +2284,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_slab3(VmrArgs<double> a) {
   }
   // loop-invariant twiddles of the passes after the first, in registers
   constexpr int NT3 = wave_tw_count<LOG2N3, 3, E>(), NT1 = wave_tw_count<LOG2N1, 3, E>();
-  constexpr bool rt3 = NT3 <= kSlabRegTw3, rt1 = NT1 <= kSlabRegTw1 && LOG2N1 == 6;
+  // the 128-point rows of the quarter kernel (11 twiddles) fit too, now that
+  // the Gaussian factors come from the axis-2 pass
+  constexpr bool rt3 = NT3 <= kSlabRegTw3,
+                 rt1 = (NT1 <= kSlabRegTw1 && LOG2N1 == 6) || (kBanksQ && NT1 <= 12);
   cplx<double> tw3r[rt3 ? NT3 : 1], tw1r[rt1 ? NT1 : 1];
   if constexpr (rt3)
     wave_tw_load<LOG2N3, 3, E, 1, double, 0, NT3>(t3, (const cplx<double>*)a.tw + n3 - 2, tw3r);
