@@ -629,6 +629,24 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
   }
 }
 
+// Pencil stride of the ladder axis pass: the padded layout, or in mode 1
+// (FP64, L >= 256) unpadded pencils with the XOR swizzle (lds_slot<1>), L + 2
+// values apart.  There a pencil's 32 or more lanes read 16 values 32 apart
+// per group in the later passes, which the padded layout serves with a 2-way
+// bank conflict in every read; swizzled they fill distinct banks, and with
+// the pencils 2 (mod 16) values apart the tile store's reads (8 columns at
+// two neighbouring rows per 16-lane group) do too.  256^2: ladder 10.79 ->
+// 10.72 ms.  At L = 128 the slot arithmetic costs the pass a wave per SIMD
+// (126 -> 154 VGPRs) for no measured gain, so shorter axes stay padded.
+template <typename T>
+constexpr int axis_ladder_swizzled(int mode, int L) {
+  return mode == 1 && sizeof(T) == 8 && L >= 256 ? 1 : 0;
+}
+template <typename T>
+constexpr int axis_ladder_stride(int mode, int L) {
+  return axis_ladder_swizzled<T>(mode, L) ? L + 2 : pencil_stride(L);
+}
+
 // Ladder form of the first inverse axis pass (grid axis 2, contiguous): the
 // block keeps TK x TJ spectrum pencils (TK columns k1, TJ outer indices j3)
 // of one field in registers -- each lane holds exactly the inputs of its
@@ -652,24 +670,6 @@ __global__ __launch_bounds__(kThreads) void k_axis_wave(AxisArgs<T> a) {
 // the n3/4-point inverse transform of c_p over k' gives the outputs
 // j3 = 4 m + p (k_slab3 with SPLIT = 4).  The arithmetic is that of
 // k_axis_wave with DIR = +1.
-// Pencil stride of the ladder axis pass: the padded layout, or in mode 1
-// (FP64, L >= 256) unpadded pencils with the XOR swizzle (lds_slot<1>), L + 2
-// values apart.  There a pencil's 32 or more lanes read 16 values 32 apart
-// per group in the later passes, which the padded layout serves with a 2-way
-// bank conflict in every read; swizzled they fill distinct banks, and with
-// the pencils 2 (mod 16) values apart the tile store's reads (8 columns at
-// two neighbouring rows per 16-lane group) do too.  256^2: ladder 10.79 ->
-// 10.72 ms.  At L = 128 the slot arithmetic costs the pass a wave per SIMD
-// (126 -> 154 VGPRs) for no measured gain, so shorter axes stay padded.
-template <typename T>
-constexpr int axis_ladder_swizzled(int mode, int L) {
-  return mode == 1 && sizeof(T) == 8 && L >= 256 ? 1 : 0;
-}
-template <typename T>
-constexpr int axis_ladder_stride(int mode, int L) {
-  return axis_ladder_swizzled<T>(mode, L) ? L + 2 : pencil_stride(L);
-}
-
 template <typename T, int LOG2L, int MODE, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<T> a) {
   constexpr int E = sizeof(T) == 8 ? 8 : 16;
