@@ -753,6 +753,12 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
       buf[(TKe + (id >> LOG2L)) * PL + lds_slot<SWZ, LE>(id & (L - 1))] = {(T)0, (T)0};
   }
 
+  // the 64-point [k1][j2][j3] pass (64^3 ladders) holds the twiddles of its
+  // second pass in registers, loaded once, instead of reading them every rung
+  constexpr bool RT = MODE == 2 && sizeof(T) == 8 && LOG2L == 6;
+  constexpr int NTW = RT ? wave_tw_count<LOG2L, LOG2R1, E>() : 1;
+  cplx<T> twr[NTW];
+  if constexpr (RT) wave_tw_load<LOG2L, LOG2R1, E, 1, T, 0, NTW>(t, a.tw + (L - 2), twr);
   for (int rr = rbeg; rr < rend; ++rr) {
     const T* gt = gbase + (int64_t)(a.r0 + rr) * a.ksum;
     // fold: the pencil's axis-3 factor (k3 = oq) times its axis-1 factor
@@ -779,7 +785,10 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_axis_ladder(AxisArgs<
       }
     }
     wave_sync();
-    wave_fft_from<LOG2L, LOG2R1, E, 1, T, SWZ>(pb, t, act, a.tw + (L - 2));
+    if constexpr (RT)
+      wave_fft_from_rt<LOG2L, LOG2R1, E, 1, T, 0, NTW>(pb, t, act, twr);
+    else
+      wave_fft_from<LOG2L, LOG2R1, E, 1, T, SWZ>(pb, t, act, a.tw + (L - 2));
     __syncthreads();
     cplx<T>* dst = a.dst + (int64_t)rr * a.dst_rs + (int64_t)(f - a.dst_fid0) * a.dst_fsize;
     if constexpr (MODE == 1) {
